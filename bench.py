@@ -1,0 +1,135 @@
+"""Throughput of the fused 2D unsplit step on one MI355X.
+
+Workload (north-star headline): 2D shallow water, Roe + entropy fix, unsplit, transverse_waves=2,
+MC limiter, CFL 0.9, periodic BCs, 4096x4096 cells on [-2.5,2.5]^2, initial h = 1 + 16 seeded
+Gaussian humps, hu = hv = 0.  One "step" = one accepted time step of the whole grid (fused step
+kernel + device-side CFL/dt controller), state resident in HBM.
+
+Prints one JSON line: cell-updates/s (FP64), roofline of the step kernel, FP32 rate, CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=500)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--n", type=int, default=4096, help="cells per side")
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--no-fp32", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-n", type=int, default=384)
+    p.add_argument("--cpu-steps", type=int, default=40)
+    return p.parse_args()
+
+
+def run_gpu(n, dtype_id, steps, warmup, seed):
+    import torch
+    from clawpack_amd import problems
+    from clawpack_amd.device import DeviceGrid, make_desc
+
+    q, dx, dy = problems.gaussian_humps(n, n, seed)
+    d = make_desc(0, dtype_id, n, n, dx, dy, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
+                  bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
+    g = DeviceGrid(d, "cuda:0")
+    g.upload(q)
+    del q
+    # dt from the initial wave speed, as after pyclaw's first accepted step
+    g.init_ctrl(0.0, 1e-4, target_steps=warmup + steps + 10**9)
+    g.advance(warmup)
+    torch.cuda.synchronize()
+    c0 = g.read_ctrl()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    e0.record(s)
+    g.advance(steps)
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    c1 = g.read_ctrl()
+    ms = e0.elapsed_time(e1)
+    accepted = int(c1.numsteps - c0.numsteps)
+    attempts = int(c1.attempts - c0.attempts)
+    assert c1.status == 0, "device status %d" % c1.status
+    return dict(ms=ms, wall=wall, accepted=accepted, attempts=attempts, cells=n * n,
+                t=c1.t, dt=c1.dt, cfl=c1.cfl)
+
+
+def cpu_baseline(n, steps, seed):
+    from clawpack_amd import problems
+    from oracle import corc
+    q, dx, dy = problems.gaussian_humps(n, n, seed)
+    d = corc.make_desc("swe", n, n, dx, dy, grav=1.0)
+    c = corc.make_ctrl(bc_lower=(2, 2), bc_upper=(2, 2))
+    corc.evolve(d, c, q, nsteps=1, dt=1e-4)  # warm the library and page tables
+    t0 = time.perf_counter()
+    _, st = corc.evolve(d, c, q, nsteps=steps, dt=1e-4)
+    el = time.perf_counter() - t0
+    return n * n * st["numsteps"] / el, el
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the multi-GPU domain decomposition is not built in this "
+              "round (every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.init()
+    r64 = run_gpu(a.n, 0, a.steps, a.warmup, a.seed)
+    ms_step = r64["ms"] / r64["accepted"]
+    value = r64["cells"] * r64["accepted"] / (r64["ms"] * 1e-3)
+    bytes_per_cell = 48.0
+    per_attempt_s = r64["ms"] * 1e-3 / r64["attempts"]
+    achieved = bytes_per_cell * r64["cells"] / per_attempt_s / 1e9
+    out = {
+        "metric": "2D SWE cell-updates/s FP64 & FP32, % HBM roofline; adjoint/forward cost",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian humps)",
+        "config": {"workload": "swe_roe_efix_unsplit_mc_periodic_humps16_%dx%d_fp64" % (a.n, a.n),
+                   "cells": r64["cells"], "accepted_steps": r64["accepted"],
+                   "attempts": r64["attempts"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "step2_fused<double,SweP<double>> + control_step per attempt",
+                     "bytes_per_cell": bytes_per_cell},
+    }
+    if not a.no_fp32:
+        r32 = run_gpu(a.n, 1, a.steps, a.warmup, a.seed)
+        v32 = r32["cells"] * r32["accepted"] / (r32["ms"] * 1e-3)
+        out["fp32"] = {"value": v32, "ms_per_step": r32["ms"] / r32["accepted"],
+                       "ratio_vs_fp64": v32 / value,
+                       "roofline_frac": 24.0 * v32 / 1e9 / HBM_PEAK_GBS}
+    if not a.no_cpu:
+        rate, el = cpu_baseline(a.cpu_n, a.cpu_steps, a.seed)
+        out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                               "sample": "C oracle (restatement of step2/flux2/rpn2/rpt2), same "
+                                         "workload at %dx%d for %d steps (%.1f s)"
+                                         % (a.cpu_n, a.cpu_n, a.cpu_steps, el)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,675 @@
+// Fused 2D unsplit wave-propagation step for gfx950 (MI355X) and its C ABI.
+//
+// Replaces, in one kernel per step, PyClaw's ClawSolver2D.step_hyperbolic -> classic step2 ->
+// flux2 -> rpn2 / limiter / rpt2 chain (upstream pyclaw/src/pyclaw/classic/{solver.py,step2.f90,
+// flux2.f90,limiter.f90}; submodule empty in the reference tree, .gitmodules:1-3), together with
+// the ghost-cell fill (solver.py _apply_bcs, done here by index mapping) and the CFL maximum.
+//
+// Kernel organisation ("marching strip"): one wavefront owns a strip of 60 columns (64 lanes:
+// 2 halo lanes each side, one column per lane) and a segment of seg_h rows, and marches up the
+// rows.  Per row it
+//   * loads the next row (coalesced, one SoA plane per component; prefetched one row ahead),
+//   * solves the y-interface below the new row in-lane (the column's history stays in VGPRs),
+//   * solves the row's x-interfaces with the left neighbour's cell data taken over the LDS
+//     crossbar (ds_bpermute), limits them with the right neighbour's wave data, splits the
+//     fluctuations transversely, and hands the left-cell share back (ds_bpermute),
+//   * finishes the y-interface one row lower (limiter needs the next y-wave) and passes the
+//     transverse y-fluxes to the neighbouring columns,
+//   * writes the cell two rows below, whose 21-cell domain of dependence is now complete.
+// Every interface is solved once per owner (halo redundancy 64/60 in x, (seg_h+4)/seg_h in y), q
+// is read once and written once per step, and no intermediate array goes to HBM.  The bound is
+// HBM bandwidth (2*num_eqn*sizeof(T) bytes per cell) or FP64 VALU throughput, whichever binds.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/clawpack_amd.h"
+#include "physics.hpp"
+
+namespace claw {
+
+constexpr int kLanes = 64;
+constexpr int kOwn = 60;  // owned columns per strip
+constexpr int kWPB = 4;   // independent wavefronts per workgroup
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+static int hip_fail(hipError_t e, const char* where) {
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return (int)e;
+}
+
+// ------------------------------------------------------------------------------------------
+// cross-lane helpers (LDS crossbar, not VALU)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double shfl(double v, int addr) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float shfl(float v, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+
+// max that lets a NaN through (so a NaN speed poisons the CFL number instead of vanishing)
+template <class T> __device__ __forceinline__ T nmax(T acc, T s) { return (s > acc || s != s) ? s : acc; }
+
+template <class T> __device__ __forceinline__ T wave_max(T v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = nmax(v, shfl(v, (lane ^ off) * 4));
+  return v;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+  atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// Ghost cells by index mapping (pyclaw _bc_lower/_bc_upper: extrap copies the edge cell,
+// periodic wraps, wall mirrors and negates the normal momentum).  x is filled before y in pyclaw,
+// which for these three rules is the same as composing the two separable maps.
+__device__ __forceinline__ void map_index(int i, int n, int lo, int hi, int& src, bool& flip) {
+  flip = false;
+  src = i;
+  if (i < 0) {
+    if (lo == CLAW_BC_PERIODIC) src = i + n;
+    else if (lo == CLAW_BC_WALL) { src = -1 - i; flip = true; }
+    else src = 0;
+  } else if (i >= n) {
+    if (hi == CLAW_BC_PERIODIC) src = i - n;
+    else if (hi == CLAW_BC_WALL) { src = 2 * n - 1 - i; flip = true; }
+    else src = n - 1;
+  }
+  src = src < 0 ? 0 : (src > n - 1 ? n - 1 : src);  // lanes far outside the grid carry unused data
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs; renumber so that each XCD gets a contiguous
+// range of strips/segments (neighbouring strips share halo lines in that XCD's L2).
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int full = nb & ~7;
+  if (b >= full) return b;
+  const int per = full >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+template <class T, class P> struct StepArgs {
+  T* q0;
+  T* q1;
+  claw2d_ctrl* ctrl;
+  double dt_fixed;
+  double dx, dy;
+  int use_ctrl;
+  int mx, my, nstrips, nsegs, seg_h;
+  int bcx_lo, bcx_hi, bcy_lo, bcy_hi, rfx, rfy;
+  int order, tw;
+  int mthlim[4];
+  typename P::Par par;
+};
+
+// Limits one interface and forms its contributions (interface frame):
+//   to the left/lower cell : LX = -r*amdq - r*fadd,  LG1/LG2 = -r/2 * B^-/B^+ (amdq + cqxx)
+//   to the right/upper cell: RX = -r*apdq + r*fadd,  RG1/RG2 = -r/2 * B^-/B^+ (apdq - cqxx)
+// (flux2.f90: qadd, fadd += cqxx/2, cqxx folded into amdq/apdq for transverse_waves = 2, gadd).
+template <class T, class P>
+__device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T lself[P::NL],
+                                             const T dotl[P::NW], const T dotr[P::NW], T r,
+                                             const StepArgs<T, P>& a, T LX[P::NEQ], T LG1[P::NEQ],
+                                             T LG2[P::NEQ], T RX[P::NEQ], T RG1[P::NEQ],
+                                             T RG2[P::NEQ]) {
+  constexpr int NEQ = P::NEQ, NW = P::NW;
+  T cq[NEQ];
+#pragma unroll
+  for (int m = 0; m < NEQ; ++m) cq[m] = T(0);
+  if (a.order != 1) {
+    T wn[NW];
+    P::dots(lself, lself, wn);
+    T coef[NW];
+#pragma unroll
+    for (int p = 0; p < NW; ++p) {
+      const T s = P::speed(f, p);
+      const T dl = dotl[p], dr = dotr[p];  // rvalues: a conditional on the array lvalues would spill them
+      const T phi = a.mthlim[p] == 0 ? T(1) : philim(wn[p], s > T(0) ? dl : dr, a.mthlim[p]);
+      const T sa = s < T(0) ? -s : s;
+      coef[p] = sa * (T(1) - sa * r) * phi;
+    }
+    P::combine(f, coef, cq);
+  }
+  const T hr = T(0.5) * r;
+#pragma unroll
+  for (int m = 0; m < NEQ; ++m) {
+    const T fa = T(0.5) * cq[m];
+    LX[m] = -r * f.am[m] - r * fa;
+    RX[m] = -r * f.ap[m] + r * fa;
+  }
+#pragma unroll
+  for (int m = 0; m < NEQ; ++m) {
+    LG1[m] = T(0);
+    LG2[m] = T(0);
+    RG1[m] = T(0);
+    RG2[m] = T(0);
+  }
+  if (a.tw > 0) {
+    T asm_[NEQ], asp_[NEQ], bm[NEQ], bp[NEQ];
+    const bool fold = a.order != 1 && a.tw == 2;
+#pragma unroll
+    for (int m = 0; m < NEQ; ++m) {
+      asm_[m] = fold ? f.am[m] + cq[m] : f.am[m];
+      asp_[m] = fold ? f.ap[m] - cq[m] : f.ap[m];
+    }
+    P::rpt2(f, asm_, bm, bp);
+#pragma unroll
+    for (int m = 0; m < NEQ; ++m) { LG1[m] = -hr * bm[m]; LG2[m] = -hr * bp[m]; }
+    P::rpt2(f, asp_, bm, bp);
+#pragma unroll
+    for (int m = 0; m < NEQ; ++m) { RG1[m] = -hr * bm[m]; RG2[m] = -hr * bp[m]; }
+  }
+}
+
+// frame (h, normal, transverse, ...) of a y-interface -> (h, hu, hv, ...)
+template <int NEQ, class T> __device__ __forceinline__ void yswap(T v[NEQ]) {
+  const T t = v[1];
+  v[1] = v[2];
+  v[2] = t;
+}
+
+template <class T, class P>
+__global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
+  constexpr int NEQ = P::NEQ, NW = P::NW, NC = P::NC, NL = P::NL;
+  using Cell = typename P::Cell;
+  using Iface = typename P::Iface;
+
+  double dt = a.dt_fixed;
+  const T* qin = a.q0;
+  T* qout = a.q1;
+  if (a.use_ctrl) {
+    if (a.ctrl->done) return;
+    dt = a.ctrl->dt;
+    if (a.ctrl->cur) { qin = a.q1; qout = a.q0; }
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave = xcd_remap(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6);
+  if (wave >= a.nstrips * a.nsegs) return;  // uniform per wavefront
+  const int strip = wave % a.nstrips, seg = wave / a.nstrips;
+  const int mx = a.mx, my = a.my;
+  const int x0 = strip * kOwn, y0 = seg * a.seg_h;
+  const int y1 = min(y0 + a.seg_h, my);
+  const size_t plane = (size_t)mx * (size_t)my;
+  const int c = x0 - 2 + lane;
+  int cx;
+  bool fx;
+  map_index(c, mx, a.bcx_lo, a.bcx_hi, cx, fx);
+  const int addrL = (lane > 0 ? lane - 1 : 0) * 4;
+  const int addrR = (lane < 63 ? lane + 1 : 63) * 4;
+  const bool own_col = lane >= 2 && lane < 2 + kOwn && c < mx;
+  const bool xcfl_col = lane >= 1 && c >= 0 && c <= mx;   // x-interfaces 1..mx+1 (1-based)
+  const bool ycfl_col = c >= -1 && c <= mx;               // y-sweeps on columns 0..mx+1
+  const T dtdx = T(dt / a.dx), dtdy = T(dt / a.dy);
+
+  auto load_row = [&](int R, T q[NEQ]) {
+    int ry;
+    bool fy;
+    map_index(R, my, a.bcy_lo, a.bcy_hi, ry, fy);
+    const size_t off = (size_t)ry * (size_t)mx + (size_t)cx;
+#pragma unroll
+    for (int m = 0; m < NEQ; ++m) {
+      const T v = qin[(size_t)m * plane + off];
+      const bool neg = (fx && m == a.rfx) != (fy && m == a.rfy);
+      q[m] = neg ? -v : v;
+    }
+  };
+
+  T smx = T(0), smy = T(0);
+  bool bad = false;
+  Cell cprev;
+  Iface fyprev;
+  T ydotl[NW];
+  T qm1[NEQ], qm2[NEQ], qnext[NEQ];
+  T acc0[NEQ], acc1[NEQ], acc2[NEQ], acc3[NEQ];
+  T yX[NEQ], yG1[NEQ], yG2[NEQ];
+#pragma unroll
+  for (int m = 0; m < NEQ; ++m) {
+    qm1[m] = qm2[m] = T(0);
+    acc0[m] = acc1[m] = acc2[m] = acc3[m] = T(0);
+    yX[m] = yG1[m] = yG2[m] = T(0);
+  }
+#pragma unroll
+  for (int p = 0; p < NW; ++p) ydotl[p] = T(0);
+
+  load_row(y0 - 2, qnext);
+  for (int R = y0 - 2; R <= y1 + 1; ++R) {
+    T qR[NEQ];
+#pragma unroll
+    for (int m = 0; m < NEQ; ++m) qR[m] = qnext[m];
+    if (R <= y1) load_row(R + 1, qnext);
+    const Cell cR = P::prep(qR, a.par);
+
+    // ---- y-interface R (between rows R-1 and R), in-lane
+    Iface fynew;
+    if (R >= y0 - 1) {
+      P::template rpn2<1>(cprev, cR, a.par, fynew);
+      if (ycfl_col && R >= 0 && R <= my) smy = nmax(smy, P::smax(fynew));
+    }
+
+    // ---- x-interfaces of row R
+    if (R >= y0 - 1 && R <= y1) {
+      T pk[NC], pl[NC];
+      P::pack(cR, pk);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) pl[k] = shfl(pk[k], addrL);
+      const Cell cL = P::unpack(pl);
+      Iface fxi;
+      P::template rpn2<0>(cL, cR, a.par, fxi);
+      if (xcfl_col && R >= -1 && R <= my) smx = nmax(smx, P::smax(fxi));
+      T lself[NL], lnb[NL], dotr[NW], dotl[NW];
+      P::lim(fxi, lself);
+#pragma unroll
+      for (int k = 0; k < NL; ++k) lnb[k] = shfl(lself[k], addrR);
+      P::dots(lself, lnb, dotr);
+#pragma unroll
+      for (int p = 0; p < NW; ++p) dotl[p] = shfl(dotr[p], addrL);
+      T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
+      finish_iface<T, P>(fxi, lself, dotl, dotr, dtdx, a, LX, LG1, LG2, RX, RG1, RG2);
+#pragma unroll
+      for (int m = 0; m < NEQ; ++m) {
+        const T X = RX[m] + shfl(LX[m], addrR);
+        const T G1 = RG1[m] + shfl(LG1[m], addrR);
+        const T G2 = RG2[m] + shfl(LG2[m], addrR);
+        acc1[m] -= dtdy * G1;
+        acc2[m] += X - dtdy * (G2 - G1);
+        acc3[m] += dtdy * G2;
+      }
+    }
+
+    // ---- finish y-interface R-1 (between rows R-2 and R-1)
+    T ydotr[NW];
+    if (R >= y0) {
+      T lp[NL], ln[NL];
+      P::lim(fyprev, lp);
+      P::lim(fynew, ln);
+      P::dots(lp, ln, ydotr);
+      if (R >= y0 + 1) {
+        T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
+        finish_iface<T, P>(fyprev, lp, ydotl, ydotr, dtdy, a, LX, LG1, LG2, RX, RG1, RG2);
+        yswap<NEQ>(LX); yswap<NEQ>(LG1); yswap<NEQ>(LG2);
+        yswap<NEQ>(RX); yswap<NEQ>(RG1); yswap<NEQ>(RG2);
+        if (R >= y0 + 2) {
+          // cell (c, R-2) has all its y-contributions
+#pragma unroll
+          for (int m = 0; m < NEQ; ++m) {
+            const T Y = yX[m] + LX[m];
+            const T G1 = yG1[m] + LG1[m];
+            const T G2 = yG2[m] + LG2[m];
+            const T g2l = shfl(G2, addrL);
+            const T g1r = shfl(G1, addrR);
+            acc0[m] += Y - dtdx * (G2 - G1);
+            acc0[m] += dtdx * g2l - dtdx * g1r;
+          }
+          if (own_col) {
+            T qn[NEQ];
+            const size_t off = (size_t)(R - 2) * (size_t)mx + (size_t)c;
+#pragma unroll
+            for (int m = 0; m < NEQ; ++m) {
+              qn[m] = qm2[m] + acc0[m];
+              qout[(size_t)m * plane + off] = qn[m];
+            }
+            bad |= P::bad(qn);
+          }
+        }
+#pragma unroll
+        for (int m = 0; m < NEQ; ++m) { yX[m] = RX[m]; yG1[m] = RG1[m]; yG2[m] = RG2[m]; }
+      }
+#pragma unroll
+      for (int p = 0; p < NW; ++p) ydotl[p] = ydotr[p];
+    }
+
+    // ---- roll the column window
+#pragma unroll
+    for (int m = 0; m < NEQ; ++m) {
+      qm2[m] = qm1[m];
+      qm1[m] = qR[m];
+      acc0[m] = acc1[m];
+      acc1[m] = acc2[m];
+      acc2[m] = acc3[m];
+      acc3[m] = T(0);
+    }
+    cprev = cR;
+    fyprev = fynew;
+  }
+
+  const T wx = wave_max(smx), wy = wave_max(smy);
+  const unsigned long long anybad = __ballot(bad);
+  if (lane == 0) {
+    atomic_max_nonneg(&a.ctrl->smax_bits[0], (double)wx);
+    atomic_max_nonneg(&a.ctrl->smax_bits[1], (double)wy);
+    if (anybad) atomicOr(&a.ctrl->status, (int)CLAW_STATUS_BAD_STATE);
+  }
+}
+
+// CFL of a fixed-dt step (classic step2 returns cfl = max over both sweeps of dt/dx*|s|).
+__global__ void finalize_step(claw2d_ctrl* c, double dt, double dx, double dy) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double sx = __longlong_as_double((long long)c->smax_bits[0]);
+  const double sy = __longlong_as_double((long long)c->smax_bits[1]);
+  c->smax_last[0] = sx;
+  c->smax_last[1] = sy;
+  c->smax_bits[0] = 0ull;
+  c->smax_bits[1] = 0ull;
+  c->cfl = fmax(fmax(0.0, (dt / dx) * sx), (dt / dy) * sy);
+}
+
+// One evolve_to_time iteration after the step (upstream pyclaw solver.py): accept/reject,
+// advance t, choose the next dt, clip it to tend.
+__global__ void control_step(claw2d_ctrl* c, double dx, double dy, double cfl_max,
+                             double cfl_desired, double dt_max) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (c->done) return;
+  const double sx = __longlong_as_double((long long)c->smax_bits[0]);
+  const double sy = __longlong_as_double((long long)c->smax_bits[1]);
+  c->smax_last[0] = sx;
+  c->smax_last[1] = sy;
+  c->smax_bits[0] = 0ull;
+  c->smax_bits[1] = 0ull;
+  const double dt = c->dt;
+  const double cfl = fmax(fmax(0.0, (dt / dx) * sx), (dt / dy) * sy);
+  const bool dtvar = (c->flags & CLAW_FLAG_DT_VARIABLE) != 0;
+  const bool to_tend = (c->flags & CLAW_FLAG_RUN_TO_TEND) != 0;
+  c->attempts += 1;
+  c->cfl = cfl;
+  // a rejected attempt is discarded whatever it produced; a non-positive depth/density or a NaN
+  // speed in an attempt that would be accepted ends the run with CLAW_STATUS_BAD_STATE
+  const bool bad = (c->status & CLAW_STATUS_BAD_STATE) || cfl != cfl;
+  c->status &= ~CLAW_STATUS_BAD_STATE;
+  if (bad && !(cfl > cfl_max && dtvar)) {
+    c->status |= CLAW_STATUS_BAD_STATE;
+    c->done = 1;
+    return;
+  }
+  if (!(cfl > cfl_max)) {
+    c->cflmax = fmax(cfl, c->cflmax);
+    c->numsteps += 1;
+    if (dtvar) c->t += dt;
+    else c->t = c->tstart + (double)c->numsteps * dt;
+    c->cur ^= 1;
+  } else {
+    c->rejected += 1;
+    if (!dtvar) {
+      c->cflmax = fmax(cfl, c->cflmax);
+      c->status |= CLAW_STATUS_CFL_FIXED_DT;
+      c->done = 1;
+      return;
+    }
+  }
+  if (to_tend ? (c->t >= c->tend) : (c->numsteps >= c->target_steps)) {
+    c->done = 1;
+    return;
+  }
+  if (c->attempts >= c->max_steps) {
+    if (c->numsteps == c->max_steps) c->status |= CLAW_STATUS_MAX_STEPS;
+    c->done = 1;
+    return;
+  }
+  double ndt = dt;
+  if (dtvar) {
+    if (cfl > 0.0) {
+      ndt = fmin(dt_max, dt * cfl_desired / cfl);
+      c->dtmin = fmin(ndt, c->dtmin);
+      c->dtmax = fmax(ndt, c->dtmax);
+    } else {
+      ndt = dt_max;
+    }
+  }
+  if (to_tend) {
+    if (c->t + ndt > c->tend && c->tstart < c->tend) ndt = c->tend - c->t;
+    if (c->tend - c->t - ndt < 1.0e-14 * c->t) ndt = c->tend - c->t;
+  }
+  c->dt = ndt;
+}
+
+// ------------------------------------------------------------------------------------------
+// Compensated FP64 sum of one component (Neumaier), fixed reduction order.
+// ------------------------------------------------------------------------------------------
+constexpr int kSumBlocks = 1024, kSumThreads = 256;
+
+__device__ __forceinline__ void two_sum_acc(double& s, double& comp, double x) {
+  const double t = s + x;
+  if (fabs(s) >= fabs(x)) comp += (s - t) + x;
+  else comp += (x - t) + s;
+  s = t;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kSumThreads) mass_partial(const T* q, size_t n, double* part) {
+  __shared__ double ss[kSumThreads], sc[kSumThreads];
+  double s = 0.0, comp = 0.0;
+  const size_t stride = (size_t)kSumBlocks * kSumThreads;
+  for (size_t i = (size_t)blockIdx.x * kSumThreads + threadIdx.x; i < n; i += stride)
+    two_sum_acc(s, comp, (double)q[i]);
+  ss[threadIdx.x] = s;
+  sc[threadIdx.x] = comp;
+  __syncthreads();
+  for (int w = kSumThreads / 2; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      double a = ss[threadIdx.x], ca = sc[threadIdx.x] + sc[threadIdx.x + w];
+      two_sum_acc(a, ca, ss[threadIdx.x + w]);
+      ss[threadIdx.x] = a;
+      sc[threadIdx.x] = ca;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ss[0];
+    part[2 * blockIdx.x + 1] = sc[0];
+  }
+}
+
+__global__ void __launch_bounds__(64) mass_final(const double* part, double* out) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0, comp = 0.0;
+  for (int b = 0; b < kSumBlocks; ++b) {
+    comp += part[2 * b + 1];
+    two_sum_acc(s, comp, part[2 * b]);
+  }
+  out[0] = s + comp;
+}
+
+// ------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------
+static int default_seg_h(const claw2d_desc* d) {
+  if (d->seg_h > 0) return d->seg_h;
+  return 128;
+}
+
+static int check_desc(const claw2d_desc* d) {
+  if (!d) return fail(-1, "null descriptor");
+  if (d->solver != CLAW_SOLVER_SHALLOW_ROE_EFIX && d->solver != CLAW_SOLVER_EULER_4WAVE)
+    return fail(-2, "unsupported solver id");
+  const int neq = d->solver == CLAW_SOLVER_SHALLOW_ROE_EFIX ? 3 : 4;
+  if (d->num_eqn != neq || d->num_waves != neq) return fail(-3, "num_eqn/num_waves do not match the solver");
+  if (d->dtype != CLAW_FP64 && d->dtype != CLAW_FP32) return fail(-4, "dtype must be FP64 or FP32");
+  if (d->mx < 2 || d->my < 2) return fail(-5, "mx and my must be >= 2");
+  if (d->num_ghost != 2) return fail(-6, "num_ghost must be 2");
+  if (d->order != 1 && d->order != 2) return fail(-7, "order must be 1 or 2");
+  if (d->transverse_waves < 0 || d->transverse_waves > 2) return fail(-8, "transverse_waves must be 0, 1 or 2");
+  for (int p = 0; p < neq; ++p)
+    if (d->mthlim[p] < 0 || d->mthlim[p] > 5) return fail(-9, "limiter id must be 0..5");
+  for (int k = 0; k < 2; ++k) {
+    const int lo = d->bc_lower[k], hi = d->bc_upper[k];
+    if (lo < 1 || lo > 3 || hi < 1 || hi > 3) return fail(-10, "bc must be extrap(1), periodic(2) or wall(3)");
+    if ((lo == CLAW_BC_PERIODIC) != (hi == CLAW_BC_PERIODIC)) return fail(-11, "periodic bc must be set on both sides");
+    if (d->reflect_index[k] < 0 || d->reflect_index[k] >= neq) return fail(-12, "reflect_index out of range");
+  }
+  if (!(d->dx > 0) || !(d->dy > 0)) return fail(-13, "dx, dy must be positive");
+  if (d->seg_h < 0) return fail(-14, "seg_h must be >= 0");
+  if ((long long)d->mx * d->my > (1LL << 31)) return fail(-15, "grid too large for 32-bit row index math");
+  return 0;
+}
+
+template <class T, class P>
+static void fill_args(StepArgs<T, P>& a, const claw2d_desc* d) {
+  a.dx = d->dx;
+  a.dy = d->dy;
+  a.mx = d->mx;
+  a.my = d->my;
+  a.seg_h = default_seg_h(d);
+  a.nstrips = (d->mx + kOwn - 1) / kOwn;
+  a.nsegs = (d->my + a.seg_h - 1) / a.seg_h;
+  a.bcx_lo = d->bc_lower[0];
+  a.bcx_hi = d->bc_upper[0];
+  a.bcy_lo = d->bc_lower[1];
+  a.bcy_hi = d->bc_upper[1];
+  a.rfx = d->reflect_index[0];
+  a.rfy = d->reflect_index[1];
+  a.order = d->order;
+  a.tw = d->transverse_waves;
+  for (int p = 0; p < 4; ++p) a.mthlim[p] = d->mthlim[p];
+}
+
+template <class T> static typename SweP<T>::Par swe_par(const claw2d_desc* d) {
+  typename SweP<T>::Par p;
+  p.g = T(d->grav);
+  p.sqg = T(std::sqrt(d->grav));
+  return p;
+}
+template <class T> static typename EulerP<T>::Par euler_par(const claw2d_desc* d) {
+  typename EulerP<T>::Par p;
+  p.gamma = T(d->gamma);
+  p.gamma1 = T(d->gamma - 1.0);
+  return p;
+}
+
+template <class T, class P>
+static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q0, void* q1,
+                       claw2d_ctrl* ctrl, int use_ctrl, double dt, hipStream_t s) {
+  StepArgs<T, P> a;
+  fill_args(a, d);
+  a.q0 = (T*)q0;
+  a.q1 = (T*)q1;
+  a.ctrl = ctrl;
+  a.use_ctrl = use_ctrl;
+  a.dt_fixed = dt;
+  a.par = par;
+  const int waves = a.nstrips * a.nsegs;
+  const int blocks = (waves + kWPB - 1) / kWPB;
+  hipLaunchKernelGGL((step2_fused<T, P>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "step2_fused launch");
+}
+
+static int dispatch_step(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, int use_ctrl,
+                         double dt, hipStream_t s) {
+  const bool swe = d->solver == CLAW_SOLVER_SHALLOW_ROE_EFIX;
+  if (d->dtype == CLAW_FP64) {
+    return swe ? launch_step<double, SweP<double>>(d, swe_par<double>(d), q0, q1, ctrl, use_ctrl, dt, s)
+               : launch_step<double, EulerP<double>>(d, euler_par<double>(d), q0, q1, ctrl, use_ctrl, dt, s);
+  }
+  return swe ? launch_step<float, SweP<float>>(d, swe_par<float>(d), q0, q1, ctrl, use_ctrl, dt, s)
+             : launch_step<float, EulerP<float>>(d, euler_par<float>(d), q0, q1, ctrl, use_ctrl, dt, s);
+}
+
+}  // namespace claw
+
+using namespace claw;
+
+extern "C" {
+
+int claw2d_validate(const claw2d_desc* d) { return check_desc(d); }
+
+size_t claw2d_state_bytes(const claw2d_desc* d) {
+  if (check_desc(d) != 0) return 0;
+  const size_t es = d->dtype == CLAW_FP64 ? sizeof(double) : sizeof(float);
+  return (size_t)d->num_eqn * (size_t)d->mx * (size_t)d->my * es;
+}
+
+int claw2d_ctrl_init(const claw2d_desc* d, claw2d_ctrl* c, double t0, double dt0, double tend,
+                     int32_t run_to_tend, long long target_steps, long long max_steps) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!c) return fail(-20, "null control block");
+  if (!(dt0 > 0)) return fail(-21, "dt must be positive");
+  std::memset(c, 0, sizeof(*c));
+  c->t = t0;
+  c->tstart = t0;
+  c->tend = tend;
+  c->flags = (run_to_tend ? CLAW_FLAG_RUN_TO_TEND : 0) | (d->dt_variable ? CLAW_FLAG_DT_VARIABLE : 0);
+  c->target_steps = target_steps;
+  c->max_steps = max_steps > 0 ? max_steps : (1LL << 62);
+  double dt = dt0;
+  if (run_to_tend) {
+    if (t0 + dt > tend && t0 < tend) dt = tend - t0;
+    if (tend - t0 - dt < 1.0e-14 * t0) dt = tend - t0;
+  }
+  c->dt = dt;
+  c->dtmin = dt;
+  c->dtmax = dt;
+  if (run_to_tend ? (t0 >= tend) : (target_steps <= 0)) c->done = 1;
+  return 0;
+}
+
+int claw2d_step(const claw2d_desc* d, const void* q_in, void* q_out, double dt, claw2d_ctrl* ctrl,
+                void* stream) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!q_in || !q_out || !ctrl) return fail(-22, "null buffer");
+  if (q_in == q_out) return fail(-23, "q_in and q_out must differ");
+  if (!(dt > 0)) return fail(-21, "dt must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  rc = dispatch_step(d, const_cast<void*>(q_in), q_out, ctrl, 0, dt, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(finalize_step, dim3(1), dim3(64), 0, s, ctrl, dt, d->dx, d->dy);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "finalize_step launch");
+}
+
+int claw2d_advance(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, long long attempts,
+                   void* stream) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!q0 || !q1 || !ctrl) return fail(-22, "null buffer");
+  if (q0 == q1) return fail(-23, "q0 and q1 must differ");
+  hipStream_t s = (hipStream_t)stream;
+  for (long long k = 0; k < attempts; ++k) {
+    rc = dispatch_step(d, q0, q1, ctrl, 1, 0.0, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(control_step, dim3(1), dim3(64), 0, s, ctrl, d->dx, d->dy, d->cfl_max,
+                       d->cfl_desired, d->dt_max);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "control_step launch");
+  }
+  return 0;
+}
+
+size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc*) { return 2 * kSumBlocks * sizeof(double); }
+
+int claw2d_mass_sum(const claw2d_desc* d, const void* q, int32_t m, double* out, void* ws, void* stream) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!q || !out || !ws) return fail(-22, "null buffer");
+  if (m < 0 || m >= d->num_eqn) return fail(-24, "component out of range");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)d->mx * d->my;
+  if (d->dtype == CLAW_FP64)
+    hipLaunchKernelGGL(mass_partial<double>, dim3(kSumBlocks), dim3(kSumThreads), 0, s,
+                       (const double*)q + (size_t)m * n, n, (double*)ws);
+  else
+    hipLaunchKernelGGL(mass_partial<float>, dim3(kSumBlocks), dim3(kSumThreads), 0, s,
+                       (const float*)q + (size_t)m * n, n, (double*)ws);
+  hipLaunchKernelGGL(mass_final, dim3(1), dim3(64), 0, s, (const double*)ws, out);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "mass_sum launch");
+}
+
+size_t claw2d_abi_struct_size(int32_t which) {
+  return which == 0 ? sizeof(claw2d_desc) : (which == 1 ? sizeof(claw2d_ctrl) : 0);
+}
+
+const char* claw2d_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+"""Device-resident grid: SoA state planes, control block and the calls into libclawpack_amd.so.
+
+Memory is owned by PyTorch (HIP device tensors); the library only receives pointers.  The state
+lives as two ping-pong buffers of shape (num_eqn, my, mx), x fastest; ``cur`` in the device
+control block says which one holds q^n, so a rejected step costs nothing to undo.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import ClawCtrl, ClawDesc, check
+
+_CTRL_BYTES = C.sizeof(ClawCtrl)
+
+
+def _stream_handle(stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)
+
+
+def make_desc(solver_id, dtype, mx, my, dx, dy, num_eqn, num_waves, order=2, transverse_waves=2,
+              mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_upper=(1, 1), reflect_index=(1, 2),
+              grav=1.0, gamma=1.4, cfl_max=1.0, cfl_desired=0.9, dt_max=1e99, dt_variable=True,
+              seg_h=0, num_ghost=2):
+    d = ClawDesc()
+    d.solver, d.dtype, d.mx, d.my = int(solver_id), int(dtype), int(mx), int(my)
+    d.num_eqn, d.num_waves, d.num_ghost = int(num_eqn), int(num_waves), int(num_ghost)
+    d.order, d.transverse_waves = int(order), int(transverse_waves)
+    lim = list(mthlim)
+    lim = lim + [lim[-1]] * (4 - len(lim))
+    for k in range(4):
+        d.mthlim[k] = int(lim[k])
+    for k in range(2):
+        d.bc_lower[k], d.bc_upper[k] = int(bc_lower[k]), int(bc_upper[k])
+        d.reflect_index[k] = int(reflect_index[k])
+    d.seg_h, d.dt_variable = int(seg_h), int(bool(dt_variable))
+    d.dx, d.dy, d.grav, d.gamma = float(dx), float(dy), float(grav), float(gamma)
+    d.cfl_max, d.cfl_desired, d.dt_max = float(cfl_max), float(cfl_desired), float(dt_max)
+    return d
+
+
+class DeviceGrid:
+    """One grid on one GPU: q ping-pong pair, control block, mass-sum workspace."""
+
+    def __init__(self, desc, device="cuda"):
+        self.lib = _lib.load()
+        check(self.lib.claw2d_validate(C.byref(desc)), "claw2d_validate")
+        if not torch.cuda.is_available():
+            raise RuntimeError("clawpack_amd needs a HIP device (torch.cuda.is_available() is False)")
+        self.desc = desc
+        self.device = torch.device(device)
+        self.torch_dtype = torch.float64 if desc.dtype == _lib.FP64 else torch.float32
+        shape = (desc.num_eqn, desc.my, desc.mx)
+        self.q = [torch.zeros(shape, dtype=self.torch_dtype, device=self.device),
+                  torch.zeros(shape, dtype=self.torch_dtype, device=self.device)]
+        nb = (_CTRL_BYTES + 7) // 8
+        self.ctrl_t = torch.zeros(nb, dtype=torch.float64, device=self.device)
+        ws = self.lib.claw2d_mass_sum_workspace_bytes(C.byref(desc))
+        self.mass_ws = torch.zeros((ws + 7) // 8, dtype=torch.float64, device=self.device)
+        self.mass_out = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self.cur = 0
+
+    # ---- control block ------------------------------------------------------------------
+    @property
+    def ctrl_ptr(self):
+        return C.c_void_p(self.ctrl_t.data_ptr())
+
+    def read_ctrl(self):
+        raw = self.ctrl_t.cpu().numpy().tobytes()[:_CTRL_BYTES]
+        return ClawCtrl.from_buffer_copy(raw)
+
+    def write_ctrl(self, c):
+        buf = np.frombuffer(bytes(c) + b"\0" * (self.ctrl_t.numel() * 8 - _CTRL_BYTES), dtype=np.float64)
+        self.ctrl_t.copy_(torch.from_numpy(buf.copy()))
+
+    def init_ctrl(self, t0, dt0, tend=None, target_steps=0, max_steps=0):
+        c = ClawCtrl()
+        run_to_tend = tend is not None
+        check(self.lib.claw2d_ctrl_init(C.byref(self.desc), C.byref(c), float(t0), float(dt0),
+                                        float(tend if run_to_tend else 0.0), int(run_to_tend),
+                                        int(target_steps), int(max_steps)), "claw2d_ctrl_init")
+        c.cur = self.cur
+        self.write_ctrl(c)
+        return c
+
+    # ---- state transfer -------------------------------------------------------------------
+    def upload(self, q_host):
+        """q_host: pyclaw layout (num_eqn, mx, my) -> device planes (num_eqn, my, mx)."""
+        q = torch.from_numpy(np.ascontiguousarray(np.asarray(q_host, dtype=np.float64)))
+        planes = q.permute(0, 2, 1).contiguous().to(self.torch_dtype)
+        self.q[self.cur].copy_(planes)
+
+    def set_planes(self, planes):
+        """planes: device or host tensor (num_eqn, my, mx)."""
+        self.q[self.cur].copy_(planes)
+
+    def planes(self):
+        return self.q[self.cur]
+
+    def download(self):
+        """Current state in pyclaw layout (num_eqn, mx, my), float64 numpy, Fortran order."""
+        t = self.q[self.cur].to(torch.float64).cpu()
+        return np.asfortranarray(t.permute(0, 2, 1).numpy())
+
+    # ---- compute --------------------------------------------------------------------------
+    def step(self, dt, stream=None):
+        """One f2py-style step2 with fixed dt (q[cur] -> q[other]); returns the CFL number.
+        The control block's smax accumulators must be zero (init_ctrl leaves them zero)."""
+        src, dst = self.q[self.cur], self.q[1 - self.cur]
+        check(self.lib.claw2d_step(C.byref(self.desc), C.c_void_p(src.data_ptr()),
+                                   C.c_void_p(dst.data_ptr()), float(dt), self.ctrl_ptr,
+                                   _stream_handle(stream)), "claw2d_step")
+        self.cur = 1 - self.cur
+        c = self.read_ctrl()
+        return c.cfl, c
+
+    def enqueue_step(self, dt, stream=None):
+        """Like step() but without reading the control block back (no host synchronisation)."""
+        src, dst = self.q[self.cur], self.q[1 - self.cur]
+        check(self.lib.claw2d_step(C.byref(self.desc), C.c_void_p(src.data_ptr()),
+                                   C.c_void_p(dst.data_ptr()), float(dt), self.ctrl_ptr,
+                                   _stream_handle(stream)), "claw2d_step")
+        self.cur = 1 - self.cur
+
+    def advance(self, attempts, stream=None):
+        """Enqueue `attempts` evolve_to_time attempts (device-side accept/reject and dt)."""
+        check(self.lib.claw2d_advance(C.byref(self.desc), C.c_void_p(self.q[0].data_ptr()),
+                                      C.c_void_p(self.q[1].data_ptr()), self.ctrl_ptr,
+                                      int(attempts), _stream_handle(stream)), "claw2d_advance")
+
+    def sync_cur(self, c=None):
+        c = self.read_ctrl() if c is None else c
+        self.cur = int(c.cur)
+        return c
+
+    def mass_sum(self, m=0, stream=None):
+        """Compensated FP64 sum of component m (deterministic order), returned as a float."""
+        check(self.lib.claw2d_mass_sum(C.byref(self.desc), C.c_void_p(self.q[self.cur].data_ptr()),
+                                       int(m), C.c_void_p(self.mass_out.data_ptr()),
+                                       C.c_void_p(self.mass_ws.data_ptr()), _stream_handle(stream)),
+              "claw2d_mass_sum")
+        return float(self.mass_out.item())

@@ -1,0 +1,78 @@
+"""Seeded initial states of the workloads (pyclaw example setups restated; used by tests, smoke, bench)."""
+import numpy as np
+
+
+def centers(lower, upper, n):
+    d = (upper - lower) / n
+    return lower + (np.arange(n) + 0.5) * d, d
+
+
+def radial_dam_break(mx=200, my=200, lower=-2.5, upper=2.5, r0=0.5, h_in=2.0, h_out=1.0):
+    """pyclaw examples/shallow_2d/radial_dam_break.py initial data (h=2 inside r<0.5, else 1)."""
+    xc, dx = centers(lower, upper, mx)
+    yc, dy = centers(lower, upper, my)
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2)
+    q = np.zeros((3, mx, my), order="F")
+    q[0] = np.where(r < r0, h_in, h_out)
+    return q, dx, dy
+
+
+def gaussian_humps(mx, my, seed, n_humps=16, lower=-2.5, upper=2.5, base=1.0,
+                   amp=(0.05, 0.5), width=(0.02, 0.2)):
+    """h = base + sum_k A_k exp(-|x-c_k|^2/(2 w_k^2)), hu = hv = 0 (distances without wrap)."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(amp[0], amp[1], n_humps)
+    W = rng.uniform(width[0], width[1], n_humps)
+    cx = rng.uniform(lower, upper, n_humps)
+    cy = rng.uniform(lower, upper, n_humps)
+    xc, dx = centers(lower, upper, mx)
+    yc, dy = centers(lower, upper, my)
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    h = np.full((mx, my), base)
+    for k in range(n_humps):
+        h += A[k] * np.exp(-((X - cx[k]) ** 2 + (Y - cy[k]) ** 2) / (2.0 * W[k] ** 2))
+    q = np.zeros((3, mx, my), order="F")
+    q[0] = h
+    return q, dx, dy
+
+
+def euler_quadrants(mx, my, seed, gamma=1.4, split=(0.5, 0.5)):
+    """Four-quadrant Riemann problem on [0,1]^2: rho~U[0.1,1.5], p~U[0.03,1.5], u,v~U[-1,1]."""
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.1, 1.5, 4)
+    p = rng.uniform(0.03, 1.5, 4)
+    u = rng.uniform(-1.0, 1.0, 4)
+    v = rng.uniform(-1.0, 1.0, 4)
+    xc, dx = centers(0.0, 1.0, mx)
+    yc, dy = centers(0.0, 1.0, my)
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    quad = (X >= split[0]).astype(int) + 2 * (Y >= split[1]).astype(int)
+    q = np.zeros((4, mx, my), order="F")
+    R, U, V, P = rho[quad], u[quad], v[quad], p[quad]
+    q[0] = R
+    q[1] = R * U
+    q[2] = R * V
+    q[3] = P / (gamma - 1.0) + 0.5 * R * (U * U + V * V)
+    return q, dx, dy
+
+
+def random_smooth_swe(mx, my, seed, amp=0.3):
+    rng = np.random.default_rng(seed)
+    q = np.zeros((3, mx, my), order="F")
+    q[0] = 1.0 + 0.5 * rng.random((mx, my))
+    q[1] = amp * rng.standard_normal((mx, my))
+    q[2] = amp * rng.standard_normal((mx, my))
+    return q
+
+
+def random_euler(mx, my, seed, gamma=1.4):
+    rng = np.random.default_rng(seed)
+    rho = 0.5 + rng.random((mx, my))
+    u = 0.5 * rng.standard_normal((mx, my))
+    v = 0.5 * rng.standard_normal((mx, my))
+    p = 0.5 + rng.random((mx, my))
+    q = np.zeros((4, mx, my), order="F")
+    q[0], q[1], q[2] = rho, rho * u, rho * v
+    q[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return q

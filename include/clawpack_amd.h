@@ -1,0 +1,129 @@
+/*
+ * clawpack_amd: C ABI of the MI355X (gfx950) 2D unsplit wave-propagation step.
+ *
+ * The entry points replace the native boundary of PyClaw's 2D classic solver:
+ *
+ *   f2py  classic2.step2(maxm, num_ghost, mx, my, qold, qnew, aux, dx, dy, dt, method, mthlim,
+ *                        ..., use_fwave, rpn2, rpt2) -> (qnew, cfl)
+ *         called from ClawSolver2D.step_hyperbolic
+ *         (upstream pyclaw/src/pyclaw/classic/solver.py and classic/step2.f90; the pyclaw
+ *          submodule is empty in the reference tree, .gitmodules:1-3; the f2py build rule is
+ *          meson.build:29-35, FP64 via .f2py_f2cmap:1)
+ *
+ * plus the evolve_to_time accept/reject/dt loop that pyclaw runs in Python around it
+ * (upstream pyclaw/src/pyclaw/solver.py), which here runs on the device.
+ *
+ * All pointers named *_dev are device pointers (HIP device memory).  `stream` is a hipStream_t
+ * (NULL = default stream).  Every function returns 0 on success, a negative value for an invalid
+ * argument and a positive HIP error code for a runtime failure; claw2d_last_error() describes the
+ * last failure of the calling thread.  No function allocates device memory: the caller owns every
+ * buffer (state planes, control block, workspaces).
+ *
+ * State layout on the device: structure of arrays, num_eqn planes of my rows of mx cells, x
+ * fastest (plane m, row j, column i at index m*mx*my + j*mx + i), element type double (FP64) or
+ * float (FP32).  Interior cells only; boundary conditions are applied by index mapping inside the
+ * kernels.  The transpose from pyclaw's q(num_eqn, mx, my) happens once at upload.
+ */
+#ifndef CLAWPACK_AMD_H
+#define CLAWPACK_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Riemann solvers: riemann.shallow_roe_with_efix_2D, riemann.euler_4wave_2D */
+enum { CLAW_SOLVER_SHALLOW_ROE_EFIX = 0, CLAW_SOLVER_EULER_4WAVE = 1 };
+enum { CLAW_FP64 = 0, CLAW_FP32 = 1 };
+/* pyclaw BC codes: BC.custom=0, extrap=1, periodic=2, wall=3 (custom is rejected) */
+enum { CLAW_BC_CUSTOM = 0, CLAW_BC_EXTRAP = 1, CLAW_BC_PERIODIC = 2, CLAW_BC_WALL = 3 };
+/* claw2d_ctrl.status bits */
+enum { CLAW_STATUS_BAD_STATE = 1, CLAW_STATUS_CFL_FIXED_DT = 2, CLAW_STATUS_MAX_STEPS = 4 };
+/* claw2d_ctrl.flags bits */
+enum { CLAW_FLAG_RUN_TO_TEND = 1, CLAW_FLAG_DT_VARIABLE = 2 };
+
+/* Problem and method description (the Solver/State attributes pyclaw hands to step2). */
+typedef struct claw2d_desc {
+  int32_t solver;           /* CLAW_SOLVER_* */
+  int32_t dtype;            /* CLAW_FP64 or CLAW_FP32 */
+  int32_t mx, my;           /* interior cells, >= 2 each */
+  int32_t num_eqn;          /* 3 (shallow water) or 4 (Euler) */
+  int32_t num_waves;        /* 3 or 4 */
+  int32_t num_ghost;        /* must be 2 */
+  int32_t order;            /* method(2): 1 or 2 */
+  int32_t transverse_waves; /* method(3): 0, 1 or 2 */
+  int32_t mthlim[4];        /* limiter id per wave: 0 none, 1 minmod, 2 superbee, 3 vanleer, 4 MC, 5 BW */
+  int32_t bc_lower[2];      /* x, y */
+  int32_t bc_upper[2];
+  int32_t reflect_index[2]; /* 0-based component negated by a wall in x, y (pyclaw default 1, 2) */
+  int32_t seg_h;            /* rows per marching segment (0 = library default) */
+  int32_t dt_variable;      /* solver.dt_variable */
+  double dx, dy;
+  double grav;              /* problem_data['grav'] (shallow water) */
+  double gamma;             /* problem_data['gamma'] (Euler) */
+  double cfl_max, cfl_desired, dt_max;
+} claw2d_desc;
+
+/* Device-resident time-loop control block (one per grid). */
+typedef struct claw2d_ctrl {
+  double t, dt, tend, tstart;
+  double cfl, cflmax, dtmin, dtmax;
+  double smax_last[2];             /* max |s| of the last attempt, x and y */
+  unsigned long long smax_bits[2]; /* running max |s| (IEEE bits of a non-negative double) */
+  long long numsteps, attempts, rejected, target_steps, max_steps;
+  int32_t cur;                     /* which state buffer holds q^n (0 or 1) */
+  int32_t done;
+  int32_t status;                  /* CLAW_STATUS_* bits */
+  int32_t flags;                   /* CLAW_FLAG_* bits */
+} claw2d_ctrl;
+
+/* Checks a descriptor; 0 if the HIP path supports it. */
+int claw2d_validate(const claw2d_desc* d);
+
+/* Bytes of one state buffer (num_eqn * mx * my * element size). */
+size_t claw2d_state_bytes(const claw2d_desc* d);
+
+/* Fills a host control block for a run starting at t0 with step dt0 (pyclaw solver.dt), either
+ * to tend (run_to_tend = 1, Solver.evolve_to_time(solution, tend)) or for target_steps accepted
+ * steps.  The dt clip to tend of evolve_to_time is applied to dt0 here. */
+int claw2d_ctrl_init(const claw2d_desc* d, claw2d_ctrl* host_ctrl, double t0, double dt0,
+                     double tend, int32_t run_to_tend, long long target_steps, long long max_steps);
+
+/* One step2 with a caller-chosen dt: q_in_dev -> q_out_dev (the f2py step2 call).  The CFL number
+ * max(dt/dx*max|s_x|, dt/dy*max|s_y|) of the step is written to ctrl_dev->cfl, the speeds to
+ * ctrl_dev->smax_last, and CLAW_STATUS_BAD_STATE is or-ed into ctrl_dev->status when a depth or
+ * density is not positive.  ctrl_dev->smax_bits must be zero on entry (claw2d_ctrl_init and every
+ * call leave it zero).  Replaces classic2.step2 (upstream classic/step2.f90). */
+int claw2d_step(const claw2d_desc* d, const void* q_in_dev, void* q_out_dev, double dt,
+                claw2d_ctrl* ctrl_dev, void* stream);
+
+/* Enqueues `attempts` step attempts of Solver.evolve_to_time (upstream pyclaw solver.py) on the
+ * ping-pong pair (q0_dev, q1_dev): each attempt runs one fused step with ctrl->dt from
+ * q[ctrl->cur] into the other buffer, then one control kernel applies accept/reject
+ * (cfl <= cfl_max), t += dt, cur ^= 1 on accept, and dt <- min(dt_max, dt*cfl_desired/cfl).
+ * Once ctrl->done is set the remaining attempts are no-ops.  No host synchronisation. */
+int claw2d_advance(const claw2d_desc* d, void* q0_dev, void* q1_dev, claw2d_ctrl* ctrl_dev,
+                   long long attempts, void* stream);
+
+/* Bytes of the workspace claw2d_mass_sum needs. */
+size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc* d);
+
+/* Compensated (Neumaier) FP64 sum of component m over the interior of q_dev (FP32 or FP64 state),
+ * in a fixed reduction order (bitwise reproducible).  Writes the sum to out_dev[0].  No
+ * counterpart in the reference (pyclaw sums with numpy in FP64). */
+int claw2d_mass_sum(const claw2d_desc* d, const void* q_dev, int32_t m, double* out_dev,
+                    void* workspace_dev, void* stream);
+
+/* sizeof(claw2d_desc) for which = 0, sizeof(claw2d_ctrl) for which = 1 (binding checks). */
+size_t claw2d_abi_struct_size(int32_t which);
+
+/* Text of the last error on the calling thread ("" if none). */
+const char* claw2d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLAWPACK_AMD_H */

@@ -1,0 +1,207 @@
+"""ctypes bindings of ``oracle/claw2d_oracle.c`` (test infrastructure only).
+
+Arrays follow pyclaw's conventions: ``q`` has shape ``(num_eqn, mx, my)`` and is passed to C in
+Fortran order (component fastest), exactly like ``state.q`` handed to the f2py ``step2``.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc2d.so")
+
+SWE, EULER = 0, 1
+BC_CUSTOM, BC_EXTRAP, BC_PERIODIC, BC_WALL = 0, 1, 2, 3
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("solver", C.c_int), ("num_eqn", C.c_int), ("num_waves", C.c_int),
+        ("num_ghost", C.c_int), ("mx", C.c_int), ("my", C.c_int), ("order", C.c_int),
+        ("transverse_waves", C.c_int), ("mthlim", C.c_int * 4), ("dx", C.c_double),
+        ("dy", C.c_double), ("grav", C.c_double), ("gamma", C.c_double),
+    ]
+
+
+class Ctrl(C.Structure):
+    _fields_ = [
+        ("dt_initial", C.c_double), ("dt_max", C.c_double), ("cfl_max", C.c_double),
+        ("cfl_desired", C.c_double), ("dt_variable", C.c_int), ("max_steps", C.c_int),
+        ("bc_lower", C.c_int * 2), ("bc_upper", C.c_int * 2), ("reflect_index", C.c_int * 2),
+    ]
+
+
+class Status(C.Structure):
+    _fields_ = [
+        ("t", C.c_double), ("dt", C.c_double), ("cflmax", C.c_double), ("dtmin", C.c_double),
+        ("dtmax", C.c_double), ("last_cfl", C.c_double), ("numsteps", C.c_long),
+        ("rejected", C.c_long), ("attempts", C.c_long), ("error", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build():
+    """Compile liborc2d.so from claw2d_oracle.c with gcc (no reference sources involved)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+        dpa = C.POINTER(C.c_double)
+        L.orc_step2.argtypes = [C.POINTER(Desc), dpa, dpa, C.c_double, dpa]
+        L.orc_fill_ghosts.argtypes = [C.POINTER(Desc), C.POINTER(Ctrl), dpa, dpa]
+        L.orc_step_hyperbolic.argtypes = [C.POINTER(Desc), C.POINTER(Ctrl), dpa, C.c_double, dpa]
+        L.orc_evolve.argtypes = [C.POINTER(Desc), C.POINTER(Ctrl), dpa, C.c_double, C.c_long,
+                                 C.POINTER(Status)]
+        L.orc_rpn2.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, dpa, dpa, dpa, dpa, dpa]
+        L.orc_rpt2.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, dpa, dpa, dpa, dpa]
+        L.orc_limiter.argtypes = [C.POINTER(Desc), C.c_int, dpa, dpa]
+        L.orc_flux2.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, C.c_double, dpa, dpa, dpa,
+                                dpa, dpa]
+        L.orc_philim.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.orc_philim.restype = C.c_double
+        L.orc_component_sum.argtypes = [C.POINTER(Desc), dpa, C.c_int]
+        L.orc_component_sum.restype = C.c_double
+        for f in ("orc_step2", "orc_step_hyperbolic", "orc_evolve", "orc_rpn2", "orc_rpt2",
+                  "orc_limiter", "orc_flux2"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_desc(solver, mx, my, dx, dy, grav=1.0, gamma=1.4, order=2, transverse_waves=2,
+              mthlim=(4, 4, 4, 4), num_ghost=2):
+    d = Desc()
+    d.solver = SWE if solver in (SWE, "swe", "shallow_roe_with_efix_2D") else EULER
+    d.num_eqn = 3 if d.solver == SWE else 4
+    d.num_waves = 3 if d.solver == SWE else 4
+    d.num_ghost = num_ghost
+    d.mx, d.my, d.order, d.transverse_waves = mx, my, order, transverse_waves
+    lim = list(mthlim) + [mthlim[-1]] * (4 - len(mthlim))
+    for k in range(4):
+        d.mthlim[k] = int(lim[k])
+    d.dx, d.dy, d.grav, d.gamma = dx, dy, grav, gamma
+    return d
+
+
+def make_ctrl(bc_lower=(1, 1), bc_upper=(1, 1), reflect_index=(1, 2), dt_initial=0.1,
+              dt_max=1e99, cfl_max=1.0, cfl_desired=0.9, dt_variable=True, max_steps=10000):
+    c = Ctrl()
+    c.dt_initial, c.dt_max, c.cfl_max, c.cfl_desired = dt_initial, dt_max, cfl_max, cfl_desired
+    c.dt_variable, c.max_steps = int(bool(dt_variable)), int(max_steps)
+    for k in range(2):
+        c.bc_lower[k], c.bc_upper[k], c.reflect_index[k] = bc_lower[k], bc_upper[k], reflect_index[k]
+    return c
+
+
+def fill_ghosts(d, c, q):
+    q = np.asfortranarray(q, dtype=np.float64)
+    g = d.num_ghost
+    qbc = np.zeros((d.num_eqn, d.mx + 2 * g, d.my + 2 * g), order="F")
+    lib().orc_fill_ghosts(C.byref(d), C.byref(c), _p(q), _p(qbc))
+    return qbc
+
+
+def step2(d, qbc, dt):
+    """One f2py-style step2 call on a ghost-filled qbc -> (qnew_bc, cfl)."""
+    qold = np.asfortranarray(qbc, dtype=np.float64)
+    qnew = np.zeros_like(qold, order="F")
+    cfl = C.c_double(0.0)
+    rc = lib().orc_step2(C.byref(d), _p(qold), _p(qnew), dt, C.byref(cfl))
+    assert rc == 0
+    return qnew, cfl.value
+
+
+def step_hyperbolic(d, c, q, dt):
+    """ClawSolver2D.step_hyperbolic on interior q -> (qnew, cfl); q is not modified."""
+    qq = np.array(q, dtype=np.float64, order="F", copy=True)
+    cfl = C.c_double(0.0)
+    rc = lib().orc_step_hyperbolic(C.byref(d), C.byref(c), _p(qq), dt, C.byref(cfl))
+    assert rc == 0
+    return qq, cfl.value
+
+
+def evolve(d, c, q, nsteps=None, tend=None, t0=0.0, dt=None):
+    """Solver.evolve_to_time semantics.  nsteps: stop after that many accepted steps."""
+    qq = np.array(q, dtype=np.float64, order="F", copy=True)
+    st = Status()
+    st.t = t0
+    st.dt = c.dt_initial if dt is None else dt
+    st.cflmax = 0.0
+    rc = lib().orc_evolve(C.byref(d), C.byref(c), _p(qq), float("inf") if tend is None else tend,
+                          int(nsteps or 0), C.byref(st))
+    assert rc == 0
+    status = {k: getattr(st, k) for k, _ in Status._fields_}
+    return qq, status
+
+
+def slice_arrays(d, n):
+    g = d.num_ghost
+    L = n + 2 * g
+    return L, g
+
+
+def rpn2(d, ixy, q1d):
+    """q1d: (num_eqn, n+2g) -> wave (meqn, mwaves, L), s (mwaves, L), amdq, apdq (meqn, L)."""
+    q1d = np.asfortranarray(q1d, dtype=np.float64)
+    L = q1d.shape[1]
+    n = L - 2 * d.num_ghost
+    wave = np.zeros((d.num_eqn, d.num_waves, L), order="F")
+    s = np.zeros((d.num_waves, L), order="F")
+    amdq = np.zeros((d.num_eqn, L), order="F")
+    apdq = np.zeros((d.num_eqn, L), order="F")
+    lib().orc_rpn2(C.byref(d), ixy, n, _p(q1d), _p(wave), _p(s), _p(amdq), _p(apdq))
+    return wave, s, amdq, apdq
+
+
+def rpt2(d, ixy, q1d, asdq):
+    q1d = np.asfortranarray(q1d, dtype=np.float64)
+    asdq = np.asfortranarray(asdq, dtype=np.float64)
+    L = q1d.shape[1]
+    n = L - 2 * d.num_ghost
+    bm = np.zeros((d.num_eqn, L), order="F")
+    bp = np.zeros((d.num_eqn, L), order="F")
+    lib().orc_rpt2(C.byref(d), ixy, n, _p(q1d), _p(asdq), _p(bm), _p(bp))
+    return bm, bp
+
+
+def limiter(d, wave, s):
+    wave = np.array(wave, dtype=np.float64, order="F", copy=True)
+    s = np.asfortranarray(s, dtype=np.float64)
+    n = wave.shape[2] - 2 * d.num_ghost
+    lib().orc_limiter(C.byref(d), n, _p(wave), _p(s))
+    return wave
+
+
+def flux2(d, ixy, q1d, dtdx):
+    q1d = np.asfortranarray(q1d, dtype=np.float64)
+    L = q1d.shape[1]
+    n = L - 2 * d.num_ghost
+    qadd = np.zeros((d.num_eqn, L), order="F")
+    fadd = np.zeros((d.num_eqn, L), order="F")
+    gadd = np.zeros((d.num_eqn, 2, L), order="F")
+    cfl = C.c_double(0.0)
+    lib().orc_flux2(C.byref(d), ixy, n, dtdx, _p(q1d), _p(qadd), _p(fadd), _p(gadd), C.byref(cfl))
+    return qadd, fadd, gadd, cfl.value
+
+
+def philim(a, b, meth):
+    return lib().orc_philim(a, b, meth)
+
+
+def component_sum(d, q, m):
+    q = np.asfortranarray(q, dtype=np.float64)
+    return lib().orc_component_sum(C.byref(d), _p(q), m)

@@ -1,0 +1,323 @@
+"""Whole-array float64 PyTorch restatement of the 2D unsplit step (test infrastructure only).
+
+Independent of ``claw2d_oracle.c``: instead of sweeping slices and updating ``qnew`` in place
+(upstream ``step2.f90``/``flux2.f90``), it evaluates every interface of a direction at once and
+then gathers the per-cell sums
+
+    qnew(i,j) = q(i,j) + X(i,j) + dtdy*G2x(i,j-1) - dtdy*G1x(i,j+1)
+                       + Y(i,j) + dtdx*G2y(i-1,j) - dtdx*G1y(i+1,j)
+
+with X = qadd - dtdx*(fadd(i+1)-fadd(i)) - dtdy*(G2x-G1x) from the x-interfaces of row j
+and Y the same for the y-interfaces of column i.  The Riemann solvers, limiter and
+transverse splitting follow the published method (LeVeque 1997; upstream
+rpn2/rpt2_shallow_roe_with_efix.f90, rpn2_euler_4wave.f90, rpt2_euler.f90, limiter.f90 —
+files absent from the reference tree, whose riemann/pyclaw submodules are empty,
+.gitmodules:1-12).
+
+Every branch is a ``torch.where`` with guarded denominators, so the function is differentiable
+and is the float64 autograd reference of the hand-written adjoint.
+"""
+import torch
+
+BC_EXTRAP, BC_PERIODIC, BC_WALL = 1, 2, 3
+
+
+def _safe(den, mask):
+    return torch.where(mask, den, torch.ones_like(den))
+
+
+def ghost_index(n, g, lower, upper):
+    """Source cell (0-based interior index) and sign flag for each padded position 0..n+2g-1."""
+    idx, flip = [], []
+    for p in range(n + 2 * g):
+        i = p - g  # 0-based interior index, may be outside [0, n)
+        if 0 <= i < n:
+            idx.append(i); flip.append(False)
+        elif i < 0:
+            if lower == BC_EXTRAP:
+                idx.append(0); flip.append(False)
+            elif lower == BC_PERIODIC:
+                idx.append(i + n); flip.append(False)
+            elif lower == BC_WALL:
+                idx.append(-1 - i); flip.append(True)
+            else:
+                raise ValueError("unsupported bc %r" % lower)
+        else:
+            if upper == BC_EXTRAP:
+                idx.append(n - 1); flip.append(False)
+            elif upper == BC_PERIODIC:
+                idx.append(i - n); flip.append(False)
+            elif upper == BC_WALL:
+                idx.append(2 * n - 1 - i); flip.append(True)
+            else:
+                raise ValueError("unsupported bc %r" % upper)
+    return torch.tensor(idx), torch.tensor(flip)
+
+
+def fill_ghosts(q, g, bc_lower, bc_upper, reflect_index=(1, 2)):
+    """q (meqn, mx, my) -> qbc (meqn, mx+2g, my+2g); separable index maps with wall sign flips."""
+    meqn, mx, my = q.shape
+    ix, fx = ghost_index(mx, g, bc_lower[0], bc_upper[0])
+    iy, fy = ghost_index(my, g, bc_lower[1], bc_upper[1])
+    qbc = q[:, ix][:, :, iy]
+    sign = torch.ones(meqn, mx + 2 * g, my + 2 * g, dtype=q.dtype, device=q.device)
+    sign[reflect_index[0], fx.to(q.device), :] *= -1.0
+    sign[reflect_index[1], :, fy.to(q.device)] *= -1.0
+    return qbc * sign
+
+
+# ---------------------------------------------------------------------------------------------
+# Riemann solvers on whole arrays.  ql, qr: (meqn, ...) states left/right of each interface;
+# components are reordered so that index 1 is the normal and 2 the transverse momentum.
+# ---------------------------------------------------------------------------------------------
+def _roe_swe(ql, qr, grav):
+    hl, hr = ql[0], qr[0]
+    sl, sr = torch.sqrt(hl), torch.sqrt(hr)
+    ssum = sl + sr
+    u = (ql[1] / sl + qr[1] / sr) / ssum
+    v = (ql[2] / sl + qr[2] / sr) / ssum
+    a = torch.sqrt(grav * ((hl + hr) * 0.5))
+    return u, v, a
+
+
+def rpn2_swe(ql, qr, grav):
+    """Returns waves (3 eqn, 3 waves, ...), speeds (3, ...), amdq, apdq (3, ...)."""
+    u, v, a = _roe_swe(ql, qr, grav)
+    d = qr - ql
+    a1 = ((u + a) * d[0] - d[1]) * (0.5 / a)
+    a2 = -v * d[0] + d[2]
+    a3 = (-(u - a) * d[0] + d[1]) * (0.5 / a)
+    z = torch.zeros_like(a1)
+    W1 = torch.stack([a1, a1 * (u - a), a1 * v])
+    W2 = torch.stack([z, z, a2])
+    W3 = torch.stack([a3, a3 * (u + a), a3 * v])
+    s1, s2, s3 = u - a, u, u + a
+    # Harten-Hyman entropy fix (left-going part)
+    s0 = ql[1] / ql[0] - torch.sqrt(grav * ql[0])
+    supersonic = (s0 >= 0) & (s1 > 0)
+    h1 = ql[0] + W1[0]
+    hu1 = ql[1] + W1[1]
+    s1r = hu1 / h1 - torch.sqrt(grav * torch.where(h1 > 0, h1, torch.ones_like(h1)))
+    t1 = (s0 < 0) & (s1r > 0)
+    sf1 = torch.where(t1, s0 * (s1r - s1) / _safe(s1r - s0, t1),
+                      torch.where(s1 < 0, s1, torch.zeros_like(s1)))
+    left2 = s2 < 0
+    s3r = qr[1] / qr[0] + torch.sqrt(grav * qr[0])
+    h3 = qr[0] - W3[0]
+    hu3 = qr[1] - W3[1]
+    s3l = hu3 / h3 + torch.sqrt(grav * torch.where(h3 > 0, h3, torch.ones_like(h3)))
+    t3 = (s3l < 0) & (s3r > 0)
+    sf3 = torch.where(t3, s3l * (s3r - s3) / _safe(s3r - s3l, t3),
+                      torch.where(s3 < 0, s3, torch.zeros_like(s3)))
+    amdq = sf1 * W1
+    amdq = amdq + torch.where(left2, s2, z) * W2
+    amdq = amdq + torch.where(left2, sf3, z) * W3
+    amdq = torch.where(supersonic, torch.zeros_like(amdq), amdq)
+    df = s1 * W1 + s2 * W2 + s3 * W3
+    apdq = df - amdq
+    waves = torch.stack([W1, W2, W3], dim=1)
+    speeds = torch.stack([s1, s2, s3])
+    return waves, speeds, amdq, apdq
+
+
+def rpt2_swe(ql, qr, asdq, grav):
+    u, v, a = _roe_swe(ql, qr, grav)
+    b1 = (0.5 / a) * ((v + a) * asdq[0] - asdq[2])
+    b2 = asdq[1] - u * asdq[0]
+    b3 = (0.5 / a) * (-(v - a) * asdq[0] + asdq[2])
+    z = torch.zeros_like(b1)
+    B = [torch.stack([b1, b1 * u, b1 * (v - a)]), torch.stack([z, b2, z]),
+         torch.stack([b3, b3 * u, b3 * (v + a)])]
+    sb = [v - a, v, v + a]
+    bm = sum(torch.clamp(s, max=0.0) * w for s, w in zip(sb, B))
+    bp = sum(torch.clamp(s, min=0.0) * w for s, w in zip(sb, B))
+    return bm, bp
+
+
+def _roe_euler(ql, qr, gamma1):
+    rl, rr = torch.sqrt(ql[0]), torch.sqrt(qr[0])
+    pl = gamma1 * (ql[3] - 0.5 * (ql[1] ** 2 + ql[2] ** 2) / ql[0])
+    pr = gamma1 * (qr[3] - 0.5 * (qr[1] ** 2 + qr[2] ** 2) / qr[0])
+    rsum = rl + rr
+    u = (ql[1] / rl + qr[1] / rr) / rsum
+    v = (ql[2] / rl + qr[2] / rr) / rsum
+    enth = ((ql[3] + pl) / rl + (qr[3] + pr) / rr) / rsum
+    u2v2 = u * u + v * v
+    a2 = gamma1 * (enth - 0.5 * u2v2)
+    a = torch.sqrt(a2)
+    return u, v, enth, u2v2, a, gamma1 / a2, enth - u2v2
+
+
+def _sound(q, gamma):
+    gamma1 = gamma - 1.0
+    p = gamma1 * (q[3] - 0.5 * (q[1] ** 2 + q[2] ** 2) / q[0])
+    return torch.sqrt(gamma * p / q[0])
+
+
+def rpn2_euler(ql, qr, gamma):
+    gamma1 = gamma - 1.0
+    u, v, enth, u2v2, a, g1a2, euv = _roe_euler(ql, qr, gamma1)
+    d = qr - ql
+    a3 = g1a2 * (euv * d[0] + u * d[1] + v * d[2] - d[3])
+    a2 = d[2] - v * d[0]
+    a4 = (d[1] + (a - u) * d[0] - a * a3) / (2.0 * a)
+    a1 = d[0] - a3 - a4
+    z = torch.zeros_like(a1)
+    W1 = torch.stack([a1, a1 * (u - a), a1 * v, a1 * (enth - u * a)])
+    W2 = torch.stack([a3, a3 * u, a3 * v, a3 * 0.5 * u2v2])
+    W3 = torch.stack([z, z, a2, a2 * v])
+    W4 = torch.stack([a4, a4 * (u + a), a4 * v, a4 * (enth + u * a)])
+    s1, s2, s3, s4 = u - a, u, u, u + a
+    s0 = ql[1] / ql[0] - _sound(ql, gamma)
+    supersonic = (s0 >= 0) & (s1 > 0)
+    q1 = ql + W1
+    s1r = q1[1] / q1[0] - _sound(q1, gamma)
+    t1 = (s0 < 0) & (s1r > 0)
+    sf1 = torch.where(t1, s0 * (s1r - s1) / _safe(s1r - s0, t1),
+                      torch.where(s1 < 0, s1, torch.zeros_like(s1)))
+    left2 = s2 < 0
+    s4r = qr[1] / qr[0] + _sound(qr, gamma)
+    q4 = qr - W4
+    s4l = q4[1] / q4[0] + _sound(q4, gamma)
+    t4 = (s4l < 0) & (s4r > 0)
+    sf4 = torch.where(t4, s4l * (s4r - s4) / _safe(s4r - s4l, t4),
+                      torch.where(s4 < 0, s4, torch.zeros_like(s4)))
+    amdq = sf1 * W1
+    amdq = amdq + torch.where(left2, s2, z) * W2
+    amdq = amdq + torch.where(left2, s3, z) * W3
+    amdq = amdq + torch.where(left2, sf4, z) * W4
+    amdq = torch.where(supersonic, torch.zeros_like(amdq), amdq)
+    df = s1 * W1 + s2 * W2 + s3 * W3 + s4 * W4
+    apdq = df - amdq
+    return torch.stack([W1, W2, W3, W4], dim=1), torch.stack([s1, s2, s3, s4]), amdq, apdq
+
+
+def rpt2_euler(ql, qr, asdq, gamma):
+    gamma1 = gamma - 1.0
+    u, v, enth, u2v2, a, g1a2, euv = _roe_euler(ql, qr, gamma1)
+    a3 = g1a2 * (euv * asdq[0] + u * asdq[1] + v * asdq[2] - asdq[3])
+    a2 = asdq[1] - u * asdq[0]
+    a4 = (asdq[2] + (a - v) * asdq[0] - a * a3) / (2.0 * a)
+    a1 = asdq[0] - a3 - a4
+    z = torch.zeros_like(a1)
+    B = [torch.stack([a1, a1 * u, a1 * (v - a), a1 * (enth - v * a)]),
+         torch.stack([a3, a3 * u, a3 * v, a3 * 0.5 * u2v2]),
+         torch.stack([z, a2, z, a2 * u]),
+         torch.stack([a4, a4 * u, a4 * (v + a), a4 * (enth + v * a)])]
+    sb = [v - a, v, v, v + a]
+    bm = sum(torch.clamp(s, max=0.0) * w for s, w in zip(sb, B))
+    bp = sum(torch.clamp(s, min=0.0) * w for s, w in zip(sb, B))
+    return bm, bp
+
+
+def philim(theta, meth):
+    if meth == 1:
+        return torch.clamp(torch.minimum(torch.ones_like(theta), theta), min=0.0)
+    if meth == 2:
+        return torch.clamp(torch.maximum(torch.clamp(2.0 * theta, max=1.0),
+                                         torch.clamp(theta, max=2.0)), min=0.0)
+    if meth == 3:
+        return (theta + theta.abs()) / (1.0 + theta.abs())
+    if meth == 4:
+        return torch.clamp(torch.minimum((1.0 + theta) / 2.0,
+                                         torch.clamp(2.0 * theta, max=2.0)), min=0.0)
+    if meth == 5:
+        return theta
+    return torch.ones_like(theta)
+
+
+# ---------------------------------------------------------------------------------------------
+# One direction: all slices at once.  Q: (meqn, N, M), interfaces along dim 1 (N = n + 2g).
+# Returns per-cell (cells 1..n of each slice, i.e. padded positions g..g+n-1):
+#   qadd, fdiff = fadd(i+1)-fadd(i), g1, g2 (all (meqn, n, M)), and smax over interfaces 1..n+1.
+# ---------------------------------------------------------------------------------------------
+def sweep(Q, r, g, solver, phys, mthlim, order=2, transverse_waves=2):
+    meqn, N, M = Q.shape
+    n = N - 2 * g
+    ql, qr = Q[:, :-1], Q[:, 1:]  # interface k between padded cells k and k+1
+    if solver == "swe":
+        waves, s, amdq, apdq = rpn2_swe(ql, qr, phys)
+    else:
+        waves, s, amdq, apdq = rpn2_euler(ql, qr, phys)
+    # 1-based interface i (between cells i-1 and i) is k = i + g - 2; interfaces 1..n+1
+    k0, k1 = g - 1, g + n  # k range [k0, k1)
+    smax = s[:, k0:k1].abs().amax() if s.numel() else torch.zeros((), dtype=Q.dtype)
+    sl = s[:, k0:k1]
+    fadd = torch.zeros(meqn, n + 1, M, dtype=Q.dtype, device=Q.device)
+    cq = torch.zeros_like(fadd)
+    if order != 1:
+        wl, wc, wr = waves[:, :, k0 - 1:k1 - 1], waves[:, :, k0:k1], waves[:, :, k0 + 1:k1 + 1]
+        wn = (wc * wc).sum(0)
+        dotl = (wl * wc).sum(0)
+        dotr = (wc * wr).sum(0)
+        nz = wn != 0
+        theta = torch.where(sl > 0, dotl, dotr) / _safe(wn, nz)
+        phis = []
+        for mw in range(waves.shape[1]):
+            lim = mthlim[mw]
+            if lim == 0:
+                phis.append(torch.ones_like(theta[mw]))
+            else:
+                phis.append(torch.where(nz[mw], philim(theta[mw], lim), torch.ones_like(theta[mw])))
+        phi = torch.stack(phis)
+        wlim = wc * phi.unsqueeze(0)
+        sa = sl.abs()
+        cq = (sa * (1.0 - sa * r)).unsqueeze(0) * wlim
+        cq = cq.sum(1)
+        fadd = 0.5 * cq
+    am = amdq[:, k0:k1]
+    ap = apdq[:, k0:k1]
+    qadd = -r * ap[:, :-1] - r * am[:, 1:]
+    fdiff = fadd[:, 1:] - fadd[:, :-1]
+    if transverse_waves == 0:
+        z = torch.zeros_like(qadd)
+        return qadd, fdiff, z, z, smax
+    if order != 1 and transverse_waves == 2:
+        am = am + cq
+        ap = ap - cq
+    qli, qri = ql[:, k0:k1], qr[:, k0:k1]
+    if solver == "swe":
+        bm1, bp1 = rpt2_swe(qli, qri, am, phys)
+        bm2, bp2 = rpt2_swe(qli, qri, ap, phys)
+    else:
+        bm1, bp1 = rpt2_euler(qli, qri, am, phys)
+        bm2, bp2 = rpt2_euler(qli, qri, ap, phys)
+    # cell i (1..n) gets imp=1 parts of interface i+1 and imp=2 parts of interface i
+    g1 = -0.5 * r * bm1[:, 1:] - 0.5 * r * bm2[:, :-1]
+    g2 = -0.5 * r * bp1[:, 1:] - 0.5 * r * bp2[:, :-1]
+    return qadd, fdiff, g1, g2, smax
+
+
+def _swap_xy(Q, solver):
+    perm = [0, 2, 1] if solver == "swe" else [0, 2, 1, 3]
+    return Q[perm].transpose(1, 2)
+
+
+def step2(qbc, dt, dx, dy, g, solver, phys, mthlim, order=2, transverse_waves=2):
+    """qbc (meqn, mx+2g, my+2g) ghost-filled -> (q interior after one step, cfl)."""
+    meqn, NX, NY = qbc.shape
+    mx, my = NX - 2 * g, NY - 2 * g
+    dtdx, dtdy = dt / dx, dt / dy
+    # x-sweeps on rows 0..my+1 (padded g-1 .. g+my)
+    Qx = qbc[:, :, g - 1:g + my + 1]
+    qa, fd, g1, g2, smx = sweep(Qx, dtdx, g, solver, phys, mthlim, order, transverse_waves)
+    X = qa - dtdx * fd - dtdy * (g2 - g1)  # (meqn, mx, my+2), column c = row c-1
+    # y-sweeps on columns 0..mx+1
+    Qy = _swap_xy(qbc[:, g - 1:g + mx + 1, :], solver)  # (meqn, NY, mx+2)
+    qa, fd, h1, h2, smy = sweep(Qy, dtdy, g, solver, phys, mthlim, order, transverse_waves)
+    Yt = qa - dtdy * fd - dtdx * (h2 - h1)
+    Y = _swap_xy(Yt, solver)          # (meqn, mx+2, my)
+    H1 = _swap_xy(h1, solver)
+    H2 = _swap_xy(h2, solver)
+    q = qbc[:, g:g + mx, g:g + my]
+    qn = (q + dtdy * g2[:, :, 0:my] + X[:, :, 1:my + 1] - dtdy * g1[:, :, 2:my + 2]
+          + dtdx * H2[:, 0:mx, :] + Y[:, 1:mx + 1, :] - dtdx * H1[:, 2:mx + 2, :])
+    cfl = torch.maximum(dtdx * smx, dtdy * smy)
+    return qn, cfl
+
+
+def step(q, dt, dx, dy, solver, phys, mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_upper=(1, 1),
+         reflect_index=(1, 2), g=2, order=2, transverse_waves=2):
+    """ClawSolver2D.step_hyperbolic on interior q (meqn, mx, my) -> (qnew, cfl)."""
+    qbc = fill_ghosts(q, g, bc_lower, bc_upper, reflect_index)
+    return step2(qbc, dt, dx, dy, g, solver, phys, mthlim, order, transverse_waves)
