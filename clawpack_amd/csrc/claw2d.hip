@@ -23,6 +23,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -113,62 +114,65 @@ template <class T, class P> struct StepArgs {
   typename P::Par par;
 };
 
-// Limits one interface and forms its contributions (interface frame):
-//   to the left/lower cell : LX = -r*amdq - r*fadd,  LG1/LG2 = -r/2 * B^-/B^+ (amdq + cqxx)
-//   to the right/upper cell: RX = -r*apdq + r*fadd,  RG1/RG2 = -r/2 * B^-/B^+ (apdq - cqxx)
-// (flux2.f90: qadd, fadd += cqxx/2, cqxx folded into amdq/apdq for transverse_waves = 2, gadd).
-template <class T, class P>
-__device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T lself[P::NL],
-                                             const T dotl[P::NW], const T dotr[P::NW], T r,
+// Limits one interface and forms its contributions (interface frame), following flux2.f90:
+//   to the left/lower cell : LX = -r*(amdq + cqxx/2),  LG1/LG2 = rt * (-r/2) * B^-/B^+ (amdq + cqxx)
+//   to the right/upper cell: RX = -r*(apdq - cqxx/2),  RG1/RG2 = rt * (-r/2) * B^-/B^+ (apdq - cqxx)
+// (qadd, fadd += cqxx/2, cqxx folded into amdq/apdq for transverse_waves = 2, gadd), where r is
+// dt/dx along the interface normal and rt the transverse dt/dy that step2.f90 applies to gadd.
+// alL/mL and alR/mR are the wave strengths of the interfaces below/above along the sweep and
+// the eigenvector products e_p(neighbour).e_p(this) the limiter needs.
+template <class T, class P, int LIM>
+__device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T alL[P::NW],
+                                             const T mL[P::NW], const T alR[P::NW],
+                                             const T mR[P::NW], T r, T rt,
                                              const StepArgs<T, P>& a, T LX[P::NEQ], T LG1[P::NEQ],
                                              T LG2[P::NEQ], T RX[P::NEQ], T RG1[P::NEQ],
                                              T RG2[P::NEQ]) {
   constexpr int NEQ = P::NEQ, NW = P::NW;
-  T cq[NEQ];
+  const bool second = a.order != 1;
+  const bool fold = second && a.tw == 2;
+  T in[NW];
+  if (second) P::inv_norm(f, in);
+  T XL[NW], XR[NW], A[NW], B[NW];
 #pragma unroll
-  for (int m = 0; m < NEQ; ++m) cq[m] = T(0);
-  if (a.order != 1) {
-    T wn[NW];
-    P::dots(lself, lself, wn);
-    T coef[NW];
-#pragma unroll
-    for (int p = 0; p < NW; ++p) {
-      const T s = P::speed(f, p);
-      const T dl = dotl[p], dr = dotr[p];  // rvalues: a conditional on the array lvalues would spill them
-      const T phi = a.mthlim[p] == 0 ? T(1) : philim(wn[p], s > T(0) ? dl : dr, a.mthlim[p]);
+  for (int p = 0; p < NW; ++p) {
+    const T s = P::speed(f, p), al = f.al[p], sf = f.sf[p];
+    const T e = sf * al, q = (s - sf) * al;
+    T k = T(0);
+    if (second) {
+      const T aL = alL[p], aR = alR[p], ml = mL[p], mr = mR[p];  // rvalues: no lvalue select
+      const T t = (s > T(0) ? aL * ml : aR * mr) * in[p];
+      const int meth = LIM > 0 ? LIM : a.mthlim[p];
       const T sa = s < T(0) ? -s : s;
-      coef[p] = sa * (T(1) - sa * r) * phi;
+      k = sa * (T(1) - sa * r) * limited(al, t, meth);
     }
-    P::combine(f, coef, cq);
+    XL[p] = -r * fma(T(0.5), k, e);
+    XR[p] = -r * fma(T(-0.5), k, q);
+    A[p] = fold ? e + k : e;
+    B[p] = fold ? q - k : q;
   }
-  const T hr = T(0.5) * r;
+  P::expand_n(f, XL, LX);
+  P::expand_n(f, XR, RX);
 #pragma unroll
-  for (int m = 0; m < NEQ; ++m) {
-    const T fa = T(0.5) * cq[m];
-    LX[m] = -r * f.am[m] - r * fa;
-    RX[m] = -r * f.ap[m] + r * fa;
-  }
-#pragma unroll
-  for (int m = 0; m < NEQ; ++m) {
-    LG1[m] = T(0);
-    LG2[m] = T(0);
-    RG1[m] = T(0);
-    RG2[m] = T(0);
-  }
+  for (int m = 0; m < NEQ; ++m) LG1[m] = LG2[m] = RG1[m] = RG2[m] = T(0);
   if (a.tw > 0) {
-    T asm_[NEQ], asp_[NEQ], bm[NEQ], bp[NEQ];
-    const bool fold = a.order != 1 && a.tw == 2;
+    T sg[NW], bA[NW], bB[NW], c1[NW], c2[NW], c3[NW], c4[NW];
+    P::tspeeds(f, sg);
+    P::tbeta(f, A, bA);
+    P::tbeta(f, B, bB);
+    const T w = T(-0.5) * r * rt;
 #pragma unroll
-    for (int m = 0; m < NEQ; ++m) {
-      asm_[m] = fold ? f.am[m] + cq[m] : f.am[m];
-      asp_[m] = fold ? f.ap[m] - cq[m] : f.ap[m];
+    for (int j = 0; j < NW; ++j) {
+      const T wm = w * vmin(sg[j], T(0)), wp = w * vmax(sg[j], T(0));
+      c1[j] = wm * bA[j];
+      c2[j] = wp * bA[j];
+      c3[j] = wm * bB[j];
+      c4[j] = wp * bB[j];
     }
-    P::rpt2(f, asm_, bm, bp);
-#pragma unroll
-    for (int m = 0; m < NEQ; ++m) { LG1[m] = -hr * bm[m]; LG2[m] = -hr * bp[m]; }
-    P::rpt2(f, asp_, bm, bp);
-#pragma unroll
-    for (int m = 0; m < NEQ; ++m) { RG1[m] = -hr * bm[m]; RG2[m] = -hr * bp[m]; }
+    P::expand_t(f, c1, LG1);
+    P::expand_t(f, c2, LG2);
+    P::expand_t(f, c3, RG1);
+    P::expand_t(f, c4, RG2);
   }
 }
 
@@ -179,7 +183,8 @@ template <int NEQ, class T> __device__ __forceinline__ void yswap(T v[NEQ]) {
   v[2] = t;
 }
 
-template <class T, class P>
+// LIM > 0: every wave uses limiter LIM (compile-time); LIM == 0: per-wave ids from a.mthlim.
+template <class T, class P, int LIM>
 __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
   constexpr int NEQ = P::NEQ, NW = P::NW, NC = P::NC, NL = P::NL;
   using Cell = typename P::Cell;
@@ -194,8 +199,9 @@ __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
     if (a.ctrl->cur) { qin = a.q1; qout = a.q0; }
   }
   const int lane = threadIdx.x & 63;
-  const int wave = xcd_remap(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6);
-  if (wave >= a.nstrips * a.nsegs) return;  // uniform per wavefront
+  // wave-uniform quantities live in SGPRs (readfirstlane), so the row loop branches on SCC
+  const int wave = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6));
+  if (wave >= a.nstrips * a.nsegs) return;
   const int strip = wave % a.nstrips, seg = wave / a.nstrips;
   const int mx = a.mx, my = a.my;
   const int x0 = strip * kOwn, y0 = seg * a.seg_h;
@@ -212,16 +218,24 @@ __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
   const bool ycfl_col = c >= -1 && c <= mx;               // y-sweeps on columns 0..mx+1
   const T dtdx = T(dt / a.dx), dtdy = T(dt / a.dy);
 
-  auto load_row = [&](int R, T q[NEQ]) {
-    int ry;
-    bool fy;
-    map_index(R, my, a.bcy_lo, a.bcy_hi, ry, fy);
-    const size_t off = (size_t)ry * (size_t)mx + (size_t)cx;
+  // Loads row R of this lane's column.  The wall sign flips are returned, not applied, so the
+  // consumer (one row later) and not the load waits for the data.
+  const bool walls = fx || a.bcy_lo == CLAW_BC_WALL || a.bcy_hi == CLAW_BC_WALL;
+  auto load_row = [&](int R, T q[NEQ], bool& fy) {
+    int ry = R;
+    fy = false;
+    if (R < 0 || R >= my) map_index(R, my, a.bcy_lo, a.bcy_hi, ry, fy);
+    ry = __builtin_amdgcn_readfirstlane(ry);
+    const T* src = qin + (size_t)ry * (size_t)mx + (size_t)cx;
+#pragma unroll
+    for (int m = 0; m < NEQ; ++m) q[m] = src[(size_t)m * plane];
+  };
+  auto apply_flips = [&](T q[NEQ], bool fy) {
+    if (!walls) return;
 #pragma unroll
     for (int m = 0; m < NEQ; ++m) {
-      const T v = qin[(size_t)m * plane + off];
       const bool neg = (fx && m == a.rfx) != (fy && m == a.rfy);
-      q[m] = neg ? -v : v;
+      q[m] = neg ? -q[m] : q[m];
     }
   };
 
@@ -229,7 +243,7 @@ __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
   bool bad = false;
   Cell cprev;
   Iface fyprev;
-  T ydotl[NW];
+  T yal2[NW], ymprev[NW];  // y: strengths of interface R-2, e-products of the pair (R-2, R-1)
   T qm1[NEQ], qm2[NEQ], qnext[NEQ];
   T acc0[NEQ], acc1[NEQ], acc2[NEQ], acc3[NEQ];
   T yX[NEQ], yG1[NEQ], yG2[NEQ];
@@ -240,14 +254,16 @@ __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
     yX[m] = yG1[m] = yG2[m] = T(0);
   }
 #pragma unroll
-  for (int p = 0; p < NW; ++p) ydotl[p] = T(0);
+  for (int p = 0; p < NW; ++p) yal2[p] = ymprev[p] = T(0);
 
-  load_row(y0 - 2, qnext);
+  bool fnext;
+  load_row(y0 - 2, qnext, fnext);
   for (int R = y0 - 2; R <= y1 + 1; ++R) {
     T qR[NEQ];
 #pragma unroll
     for (int m = 0; m < NEQ; ++m) qR[m] = qnext[m];
-    if (R <= y1) load_row(R + 1, qnext);
+    apply_flips(qR, fnext);
+    if (R <= y1) load_row(R + 1, qnext, fnext);
     const Cell cR = P::prep(qR, a.par);
 
     // ---- y-interface R (between rows R-1 and R), in-lane
@@ -267,36 +283,40 @@ __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
       Iface fxi;
       P::template rpn2<0>(cL, cR, a.par, fxi);
       if (xcfl_col && R >= -1 && R <= my) smx = nmax(smx, P::smax(fxi));
-      T lself[NL], lnb[NL], dotr[NW], dotl[NW];
+      T lself[NL], lnb[NL], mR[NW], mL[NW], alR[NW], alL[NW];
       P::lim(fxi, lself);
 #pragma unroll
       for (int k = 0; k < NL; ++k) lnb[k] = shfl(lself[k], addrR);
-      P::dots(lself, lnb, dotr);
+      P::mdot(lself, lnb, mR);
 #pragma unroll
-      for (int p = 0; p < NW; ++p) dotl[p] = shfl(dotr[p], addrL);
+      for (int p = 0; p < NW; ++p) {
+        alR[p] = lnb[p];
+        alL[p] = shfl(fxi.al[p], addrL);
+        mL[p] = P::unit_m(p) ? T(1) : shfl(mR[p], addrL);
+      }
       T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-      finish_iface<T, P>(fxi, lself, dotl, dotr, dtdx, a, LX, LG1, LG2, RX, RG1, RG2);
+      finish_iface<T, P, LIM>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2);
 #pragma unroll
       for (int m = 0; m < NEQ; ++m) {
         const T X = RX[m] + shfl(LX[m], addrR);
         const T G1 = RG1[m] + shfl(LG1[m], addrR);
         const T G2 = RG2[m] + shfl(LG2[m], addrR);
-        acc1[m] -= dtdy * G1;
-        acc2[m] += X - dtdy * (G2 - G1);
-        acc3[m] += dtdy * G2;
+        acc1[m] -= G1;
+        acc2[m] += (X + G1) - G2;
+        acc3[m] += G2;
       }
     }
 
     // ---- finish y-interface R-1 (between rows R-2 and R-1)
-    T ydotr[NW];
     if (R >= y0) {
-      T lp[NL], ln[NL];
+      T lp[NL], ln[NL], mR[NW];
       P::lim(fyprev, lp);
       P::lim(fynew, ln);
-      P::dots(lp, ln, ydotr);
+      P::mdot(lp, ln, mR);
       if (R >= y0 + 1) {
         T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-        finish_iface<T, P>(fyprev, lp, ydotl, ydotr, dtdy, a, LX, LG1, LG2, RX, RG1, RG2);
+        finish_iface<T, P, LIM>(fyprev, yal2, ymprev, fynew.al, mR, dtdy, dtdx, a, LX, LG1, LG2,
+                                RX, RG1, RG2);
         yswap<NEQ>(LX); yswap<NEQ>(LG1); yswap<NEQ>(LG2);
         yswap<NEQ>(RX); yswap<NEQ>(RG1); yswap<NEQ>(RG2);
         if (R >= y0 + 2) {
@@ -308,8 +328,7 @@ __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
             const T G2 = yG2[m] + LG2[m];
             const T g2l = shfl(G2, addrL);
             const T g1r = shfl(G1, addrR);
-            acc0[m] += Y - dtdx * (G2 - G1);
-            acc0[m] += dtdx * g2l - dtdx * g1r;
+            acc0[m] += ((Y + G1) - G2) + (g2l - g1r);
           }
           if (own_col) {
             T qn[NEQ];
@@ -326,7 +345,10 @@ __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
         for (int m = 0; m < NEQ; ++m) { yX[m] = RX[m]; yG1[m] = RG1[m]; yG2[m] = RG2[m]; }
       }
 #pragma unroll
-      for (int p = 0; p < NW; ++p) ydotl[p] = ydotr[p];
+      for (int p = 0; p < NW; ++p) {
+        ymprev[p] = mR[p];
+        yal2[p] = fyprev.al[p];
+      }
     }
 
     // ---- roll the column window
@@ -482,9 +504,26 @@ __global__ void __launch_bounds__(64) mass_final(const double* part, double* out
 // ------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------
-static int default_seg_h(const claw2d_desc* d) {
+// Resident wavefronts of a kernel on the current device (workgroups per CU x CUs x waves per
+// workgroup), from the occupancy its register use allows.
+static long resident_waves(const void* kern) {
+  int dev = 0, cus = 0, bpc = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, kLanes * kWPB, 0) != hipSuccess) return 0;
+  return (long)cus * bpc * kWPB;
+}
+
+// Rows per marching segment.  A wavefront's run time is proportional to its rows, so the grid
+// is cut into as many segments as fit in one resident round (every wavefront starts at once
+// and ends at once); an explicit desc->seg_h overrides this.
+static int choose_seg_h(const claw2d_desc* d, long resident) {
   if (d->seg_h > 0) return d->seg_h;
-  return 128;
+  const long nstrips = (d->mx + kOwn - 1) / kOwn;
+  long nsegs = resident > 0 ? resident / nstrips : 32;
+  const long min_rows = 16;
+  nsegs = std::max(1L, std::min(nsegs, ((long)d->my + min_rows - 1) / min_rows));
+  return (int)((d->my + nsegs - 1) / nsegs);
 }
 
 static int check_desc(const claw2d_desc* d) {
@@ -513,12 +552,12 @@ static int check_desc(const claw2d_desc* d) {
 }
 
 template <class T, class P>
-static void fill_args(StepArgs<T, P>& a, const claw2d_desc* d) {
+static void fill_args(StepArgs<T, P>& a, const claw2d_desc* d, long resident) {
   a.dx = d->dx;
   a.dy = d->dy;
   a.mx = d->mx;
   a.my = d->my;
-  a.seg_h = default_seg_h(d);
+  a.seg_h = choose_seg_h(d, resident);
   a.nstrips = (d->mx + kOwn - 1) / kOwn;
   a.nsegs = (d->my + a.seg_h - 1) / a.seg_h;
   a.bcx_lo = d->bc_lower[0];
@@ -548,8 +587,16 @@ template <class T> static typename EulerP<T>::Par euler_par(const claw2d_desc* d
 template <class T, class P>
 static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q0, void* q1,
                        claw2d_ctrl* ctrl, int use_ctrl, double dt, hipStream_t s) {
+  bool same = true;
+  for (int p = 1; p < P::NW; ++p) same = same && d->mthlim[p] == d->mthlim[0];
+  const int lim = same ? d->mthlim[0] : -1;
+  const void* kern = lim == 4 ? (const void*)step2_fused<T, P, 4>
+                   : (lim == 1 ? (const void*)step2_fused<T, P, 1> : (const void*)step2_fused<T, P, 0>);
+  static thread_local long cached[3] = {-1, -1, -1};
+  const int slot = lim == 4 ? 0 : (lim == 1 ? 1 : 2);
+  if (cached[slot] < 0) cached[slot] = resident_waves(kern);
   StepArgs<T, P> a;
-  fill_args(a, d);
+  fill_args(a, d, cached[slot]);
   a.q0 = (T*)q0;
   a.q1 = (T*)q1;
   a.ctrl = ctrl;
@@ -558,7 +605,12 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   a.par = par;
   const int waves = a.nstrips * a.nsegs;
   const int blocks = (waves + kWPB - 1) / kWPB;
-  hipLaunchKernelGGL((step2_fused<T, P>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
+  if (lim == 4)
+    hipLaunchKernelGGL((step2_fused<T, P, 4>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
+  else if (lim == 1)
+    hipLaunchKernelGGL((step2_fused<T, P, 1>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
+  else
+    hipLaunchKernelGGL((step2_fused<T, P, 0>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : hip_fail(e, "step2_fused launch");
 }

@@ -1,16 +1,29 @@
 // Riemann-solver policies for the fused 2D step (gfx950).
 //
-// Each policy restates one Clawpack Riemann plugin in a compact form suited to a register-resident
-// stencil: a wave is kept as its strength alpha_p plus the few Roe averages that define its
-// eigenvector, so neighbouring interfaces exchange 6 (shallow water) or 8 (Euler) numbers for the
-// limiter instead of num_eqn*num_waves wave components.
+// Each policy restates one Clawpack Riemann plugin in "wave-scalar" form: at an interface the
+// jump is split into waves alpha_p * e_p along the Roe eigenvectors e_p, and everything the
+// step needs afterwards is a per-wave scalar times e_p:
+//
+//   amdq = sum_p sf_p alpha_p e_p          (sf_p: left-going speed, entropy fixed)
+//   apdq = sum_p (s_p - sf_p) alpha_p e_p
+//   cqxx = sum_p |s_p|(1 - |s_p| dt/dx) phi_p alpha_p e_p
+//
+// and the transverse split of sum_p A_p e_p (rpt2 applied to amdq+cqxx or apdq-cqxx) has the
+// closed form beta = M A with M = L_t E, the transverse left eigenvectors applied to the normal
+// right eigenvectors (for both systems M has entries 1/2, +-a, +-1/(2a) and an identity block).
+// The results equal upstream's component-wise evaluation up to rounding.
 //
 //   SweP   <- riemann/src/rpn2_shallow_roe_with_efix.f90, rpt2_shallow_roe_with_efix.f90
 //   EulerP <- riemann/src/rpn2_euler_4wave.f90, rpt2_euler.f90
 // (upstream riemann submodule; empty in the reference tree, .gitmodules:10-12).
 //
-// All vectors inside a policy are in the interface frame (h, normal, transverse[, E]); DIR = 0 is
-// an x-interface (normal = component 1), DIR = 1 a y-interface (normal = component 2).
+// Limiter (upstream limiter.f90/philim): theta_p = W_p(upwind).W_p / W_p.W_p.  With
+// W = alpha e, theta = t/alpha where t = alpha_upwind (e_upwind.e)/(e.e), and for minmod, MC and
+// superbee phi(theta)*alpha is a positively homogeneous function of (alpha, t) that needs no
+// division by alpha; e.e >= 1 so its reciprocal is always well conditioned.
+//
+// Vectors are in the interface frame (h, normal, transverse[, E]); DIR = 0 is an x-interface
+// (normal = component 1), DIR = 1 a y-interface (normal = component 2).
 //
 // Entropy fix (Harten-Hyman, as upstream): the transonic tests "u - c > 0 right of wave 1" and
 // "u + c < 0 left of the last wave" are evaluated in a division- and sqrt-free squared form; the
@@ -22,41 +35,81 @@
 
 namespace claw {
 
-template <class T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
-template <class T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+// Reciprocal and reciprocal square root of a positive normal number: the hardware estimate
+// (v_rcp_f64 / v_rsq_f64, about 23 correct bits) refined by two Newton-Raphson steps, which
+// leaves about 1 ulp of error instead of the correctly rounded IEEE sequences.  FP32 uses the
+// hardware instructions directly (about 1 ulp).
+__device__ __forceinline__ double frcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-(x * y), y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-(x * y), y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// philim of limiter.f90 with r = dot/wnorm2; ids as pyclaw/limiters/tvd.py (minmod 1,
-// superbee 2, van Leer 3, MC 4, beam-warming 5).  wnorm2 == 0 leaves the wave unlimited.
-template <class T> __device__ __forceinline__ T philim(T wn, T dot, int meth) {
-  if (!(wn != T(0))) return T(1);
-  const T r = dot / wn;
+__device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+
+// phi(t/alpha)*alpha for limiter `meth` (ids of pyclaw/limiters/tvd.py: minmod 1, superbee 2,
+// van Leer 3, MC 4, beam-warming 5, 0 = unlimited).  alpha == 0 gives 0 (upstream skips a wave
+// of zero norm, which is the zero wave).
+template <class T> __device__ __forceinline__ T limited(T al, T t, int meth) {
   switch (meth) {
-    case 1: return tmax(T(0), tmin(T(1), r));
-    case 2: return tmax(T(0), tmax(tmin(T(1), T(2) * r), tmin(T(2), r)));
-    case 3: { const T ar = r < T(0) ? -r : r; return (r + ar) / (T(1) + ar); }
-    case 4: return tmax(T(0), tmin((T(1) + r) * T(0.5), tmin(T(2), T(2) * r)));
-    case 5: return r;
-    default: return T(1);
+    case 1: {  // minmod: max(0, min(1, theta))
+      const T lo = vmin(al, t), hi = vmax(al, t);
+      return al > T(0) ? vmax(T(0), lo) : vmin(T(0), hi);
+    }
+    case 2: {  // superbee: max(0, min(1, 2 theta), min(2, theta))
+      const T t2 = T(2) * t, a2 = T(2) * al;
+      const T lo = vmax(vmin(al, t2), vmin(a2, t)), hi = vmin(vmax(al, t2), vmax(a2, t));
+      return al > T(0) ? vmax(T(0), lo) : vmin(T(0), hi);
+    }
+    case 3: {  // van Leer: (theta + |theta|)/(1 + |theta|)
+      if (al == T(0)) return T(0);
+      const T th = t / al, ath = th < T(0) ? -th : th;
+      return al * ((th + ath) / (T(1) + ath));
+    }
+    case 4: {  // MC: max(0, min((1 + theta)/2, 2, 2 theta))
+      const T c = T(0.5) * (al + t), t2 = T(2) * t, a2 = T(2) * al;
+      const T lo = vmin(vmin(c, t2), a2), hi = vmax(vmax(c, t2), a2);
+      return al > T(0) ? vmax(T(0), lo) : vmin(T(0), hi);
+    }
+    case 5: return al == T(0) ? T(0) : t;  // beam-warming: theta
+    default: return al;
   }
 }
 
 // ------------------------------------------------------------------------------------------
-// Shallow water, Roe + entropy fix.
+// Shallow water, Roe + entropy fix.  Waves e1 = (1, u-a, v), e2 = (0, 0, 1), e3 = (1, u+a, v);
+// transverse eigenvectors f1 = (1, u, v-a), f2 = (0, 1, 0), f3 = (1, u, v+a).
 // ------------------------------------------------------------------------------------------
 template <class T> struct SweP {
   static constexpr int NEQ = 3, NW = 3, NC = 7, NL = 6;
+  // e_p(A).e_p(B) is identically 1 for the shear wave
+  __device__ __forceinline__ static constexpr bool unit_m(int p) { return p == 1; }
   struct Par { T g, sqg; };
   // per-cell: q, sqrt(h), 1/sqrt(h), hu/sqrt(h), hv/sqrt(h)
   struct Cell { T q[3]; T s, rs, P, Q; };
-  struct Iface { T al[3]; T u, v, a, ha; T am[3], ap[3]; };
+  struct Iface { T al[3]; T sf[3]; T u, v, a, ha; };
 
   __device__ __forceinline__ static bool bad(const T q[3]) { return !(q[0] > T(0)); }
 
   __device__ __forceinline__ static Cell prep(const T q[3], const Par&) {
     Cell c;
     c.q[0] = q[0]; c.q[1] = q[1]; c.q[2] = q[2];
-    c.s = sqrt(q[0]);
-    c.rs = T(1) / c.s;
+    c.rs = frsq(q[0]);
+    c.s = q[0] * c.rs;
     c.P = q[1] * c.rs;
     c.Q = q[2] * c.rs;
     return c;
@@ -75,43 +128,41 @@ template <class T> struct SweP {
     constexpr int mu = DIR == 0 ? 1 : 2, mv = DIR == 0 ? 2 : 1;
     const T pnL = DIR == 0 ? L.P : L.Q, ptL = DIR == 0 ? L.Q : L.P;
     const T pnR = DIR == 0 ? R.P : R.Q, ptR = DIR == 0 ? R.Q : R.P;
-    const T inv = T(1) / (L.s + R.s);
+    const T inv = frcp(L.s + R.s);
     const T u = (pnL + pnR) * inv;
     const T v = (ptL + ptR) * inv;
-    const T a = sqrt(pr.g * ((L.q[0] + R.q[0]) * T(0.5)));
-    const T ha = T(0.5) / a;
+    const T gh = pr.g * ((L.q[0] + R.q[0]) * T(0.5));
+    const T ra = frsq(gh);
+    const T a = gh * ra;
+    const T ha = T(0.5) * ra;
     const T d0 = R.q[0] - L.q[0], dn = R.q[mu] - L.q[mu], dt = R.q[mv] - L.q[mv];
     const T s1 = u - a, s3 = u + a;
     const T a1 = (s3 * d0 - dn) * ha;
     const T a2 = dt - v * d0;
     const T a3 = (dn - s1 * d0) * ha;
-    T sf1 = T(0), sf2 = T(0), sf3 = T(0);
+    T sf1 = vmin(s1, T(0)), sf2 = T(0), sf3 = T(0);
     const T s0 = pnL * L.rs - pr.sqg * L.s;  // u - c in the left cell
-    if (!(s0 >= T(0) && s1 > T(0))) {
-      sf1 = s1 < T(0) ? s1 : T(0);
-      if (s0 < T(0)) {
-        const T h1 = L.q[0] + a1, hu1 = L.q[mu] + a1 * s1;
-        if (h1 > T(0) && hu1 > T(0) && hu1 * hu1 > pr.g * (h1 * h1) * h1) {
-          const T s1r = hu1 / h1 - sqrt(pr.g * h1);  // u - c right of the 1-wave
-          sf1 = s0 * (s1r - s1) / (s1r - s0);
-        }
+    if (s0 < T(0)) {
+      const T h1 = L.q[0] + a1, hu1 = L.q[mu] + a1 * s1;
+      if (h1 > T(0) && hu1 > T(0) && hu1 * hu1 > pr.g * (h1 * h1) * h1) {
+        const T s1r = hu1 / h1 - sqrt(pr.g * h1);  // u - c right of the 1-wave
+        sf1 = s0 * (s1r - s1) / (s1r - s0);
       }
-      if (u < T(0)) {
-        sf2 = u;
-        sf3 = s3 < T(0) ? s3 : T(0);
+    }
+    if (u < T(0)) {
+      sf2 = u;
+      sf3 = vmin(s3, T(0));
+      const T h3 = R.q[0] - a3, hu3 = R.q[mu] - a3 * s3;
+      if (h3 > T(0) && hu3 < T(0) && hu3 * hu3 > pr.g * (h3 * h3) * h3) {
         const T s3r = pnR * R.rs + pr.sqg * R.s;  // u + c in the right cell
-        const T h3 = R.q[0] - a3, hu3 = R.q[mu] - a3 * s3;
-        if (s3r > T(0) && h3 > T(0) && hu3 < T(0) && hu3 * hu3 > pr.g * (h3 * h3) * h3) {
+        if (s3r > T(0)) {
           const T s3l = hu3 / h3 + sqrt(pr.g * h3);  // u + c left of the 3-wave
           sf3 = s3l * (s3r - s3) / (s3r - s3l);
         }
       }
     }
-    const T e1 = sf1 * a1, e2 = sf2 * a2, e3 = sf3 * a3;
-    const T p1 = (s1 - sf1) * a1, p2 = (u - sf2) * a2, p3 = (s3 - sf3) * a3;
-    f.am[0] = e1 + e3; f.am[1] = e1 * s1 + e3 * s3; f.am[2] = (e1 + e3) * v + e2;
-    f.ap[0] = p1 + p3; f.ap[1] = p1 * s1 + p3 * s3; f.ap[2] = (p1 + p3) * v + p2;
     f.al[0] = a1; f.al[1] = a2; f.al[2] = a3;
+    f.sf[0] = sf1; f.sf[1] = sf2; f.sf[2] = sf3;
     f.u = u; f.v = v; f.a = a; f.ha = ha;
   }
 
@@ -119,57 +170,76 @@ template <class T> struct SweP {
   __device__ __forceinline__ static T speed(const Iface& f, int p) {
     return p == 0 ? f.u - f.a : (p == 1 ? f.u : f.u + f.a);
   }
-  // limiter data: alpha1..3, s1, s3, v
+  // limiter data sent to neighbours: alpha1..3 and the eigenvector parameters s1, s3, v
   __device__ __forceinline__ static void lim(const Iface& f, T o[NL]) {
     o[0] = f.al[0]; o[1] = f.al[1]; o[2] = f.al[2]; o[3] = f.u - f.a; o[4] = f.u + f.a; o[5] = f.v;
   }
-  __device__ __forceinline__ static void dots(const T A[NL], const T B[NL], T d[NW]) {
-    const T vv = A[5] * B[5];
-    d[0] = (A[0] * B[0]) * ((T(1) + A[3] * B[3]) + vv);
-    d[1] = A[1] * B[1];
-    d[2] = (A[2] * B[2]) * ((T(1) + A[4] * B[4]) + vv);
+  // m_p = e_p(A).e_p(B)
+  __device__ __forceinline__ static void mdot(const T A[NL], const T B[NL], T m[NW]) {
+    const T vv1 = fma(A[5], B[5], T(1));
+    m[0] = fma(A[3], B[3], vv1);
+    m[1] = T(1);
+    m[2] = fma(A[4], B[4], vv1);
   }
-  // sum_p coef_p * alpha_p * e_p  (frame vector)
-  __device__ __forceinline__ static void combine(const Iface& f, const T coef[NW], T o[NEQ]) {
-    const T k1 = coef[0] * f.al[0], k2 = coef[1] * f.al[1], k3 = coef[2] * f.al[2];
-    o[0] = k1 + k3;
-    o[1] = k1 * (f.u - f.a) + k3 * (f.u + f.a);
-    o[2] = (k1 + k3) * f.v + k2;
+  __device__ __forceinline__ static void inv_norm(const Iface& f, T in[NW]) {
+    const T vv1 = fma(f.v, f.v, T(1));
+    const T s1 = f.u - f.a, s3 = f.u + f.a;
+    const T n1 = fma(s1, s1, vv1), n3 = fma(s3, s3, vv1);
+    const T r = frcp(n1 * n3);
+    in[0] = n3 * r;
+    in[1] = T(1);
+    in[2] = n1 * r;
   }
-  __device__ __forceinline__ static void rpt2(const Iface& f, const T as[NEQ], T bm[NEQ], T bp[NEQ]) {
-    const T sg1 = f.v - f.a, sg3 = f.v + f.a;
-    const T b1 = (sg3 * as[0] - as[2]) * f.ha;
-    const T b2 = as[1] - f.u * as[0];
-    const T b3 = (as[2] - sg1 * as[0]) * f.ha;
-    const T m1 = tmin(sg1, T(0)) * b1, m2 = tmin(f.v, T(0)) * b2, m3 = tmin(sg3, T(0)) * b3;
-    const T q1 = tmax(sg1, T(0)) * b1, q2 = tmax(f.v, T(0)) * b2, q3 = tmax(sg3, T(0)) * b3;
-    bm[0] = m1 + m3; bm[1] = (m1 + m3) * f.u + m2; bm[2] = m1 * sg1 + m3 * sg3;
-    bp[0] = q1 + q3; bp[1] = (q1 + q3) * f.u + q2; bp[2] = q1 * sg1 + q3 * sg3;
+  // sum_p c_p e_p
+  __device__ __forceinline__ static void expand_n(const Iface& f, const T c[NW], T o[NEQ]) {
+    o[0] = c[0] + c[2];
+    o[1] = c[0] * (f.u - f.a) + c[2] * (f.u + f.a);
+    o[2] = fma(o[0], f.v, c[1]);
+  }
+  // transverse strengths of sum_p A_p e_p
+  __device__ __forceinline__ static void tbeta(const Iface& f, const T A[NW], T b[NW]) {
+    const T m = T(0.5) * (A[0] + A[2]), d = A[1] * f.ha;
+    b[0] = m - d;
+    b[1] = f.a * (A[2] - A[0]);
+    b[2] = m + d;
+  }
+  __device__ __forceinline__ static void tspeeds(const Iface& f, T sg[NW]) {
+    sg[0] = f.v - f.a; sg[1] = f.v; sg[2] = f.v + f.a;
+  }
+  // sum_j c_j f_j
+  __device__ __forceinline__ static void expand_t(const Iface& f, const T c[NW], T o[NEQ]) {
+    o[0] = c[0] + c[2];
+    o[1] = fma(o[0], f.u, c[1]);
+    o[2] = c[0] * (f.v - f.a) + c[2] * (f.v + f.a);
   }
 };
 
 // ------------------------------------------------------------------------------------------
 // Euler, 4 waves (u-c, entropy u, shear u, u+c), Roe + entropy fix on waves 1 and 4.
+// e1 = (1, u-a, v, H-ua), e2 = (1, u, v, q2/2), e3 = (0, 0, 1, v), e4 = (1, u+a, v, H+ua);
+// f1 = (1, u, v-a, H-va), f2 = (1, u, v, q2/2), f3 = (0, 1, 0, u), f4 = (1, u, v+a, H+va).
 // ------------------------------------------------------------------------------------------
 template <class T> struct EulerP {
   static constexpr int NEQ = 4, NW = 4, NC = 10, NL = 8;
+  __device__ __forceinline__ static constexpr bool unit_m(int) { return false; }
   struct Par { T gamma, gamma1; };
   // per-cell: q, sqrt(rho), 1/sqrt(rho), rho u/sqrt(rho), rho v/sqrt(rho), (E+p)/sqrt(rho), c
   struct Cell { T q[4]; T s, rs, P, Q, Hs, c; };
-  struct Iface { T al[4]; T u, v, a, H, u2v2, g1a2, euv, i2a; T am[4], ap[4]; };
+  struct Iface { T al[4]; T sf[4]; T u, v, a, H, i2a; };
 
   __device__ __forceinline__ static bool bad(const T q[4]) { return !(q[0] > T(0)); }
 
   __device__ __forceinline__ static Cell prep(const T q[4], const Par& pr) {
     Cell c;
     c.q[0] = q[0]; c.q[1] = q[1]; c.q[2] = q[2]; c.q[3] = q[3];
-    c.s = sqrt(q[0]);
-    c.rs = T(1) / c.s;
+    c.rs = frsq(q[0]);
+    c.s = q[0] * c.rs;
     c.P = q[1] * c.rs;
     c.Q = q[2] * c.rs;
-    const T p = pr.gamma1 * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+    const T p = pr.gamma1 * (q[3] - T(0.5) * (c.P * c.P + c.Q * c.Q));
     c.Hs = (q[3] + p) * c.rs;
-    c.c = sqrt(pr.gamma * p / q[0]);
+    const T c2 = pr.gamma * p * (c.rs * c.rs);
+    c.c = c2 * frsq(c2);
     return c;
   }
   __device__ __forceinline__ static void pack(const Cell& c, T o[NC]) {
@@ -183,8 +253,8 @@ template <class T> struct EulerP {
     return c;
   }
 
-  // squared-form test of "n/rho - c > 0" (sign = +1) or "n/rho + c < 0" (sign = -1) for the
-  // state (rho, n, t, E); P = p*rho.
+  // squared-form test of "n/rho - c > 0" (right = true) or "n/rho + c < 0" (right = false) for
+  // the state (rho, n, t, E), using p*rho = gamma1*(E*rho - (n^2+t^2)/2).
   __device__ __forceinline__ static bool sonic_cross(T rho, T n, T t, T E, const Par& pr, bool right) {
     const T Prho = pr.gamma1 * (E * rho - T(0.5) * (n * n + t * t));
     const bool dir = right ? (n > T(0)) : (n < T(0));
@@ -200,16 +270,17 @@ template <class T> struct EulerP {
     constexpr int mu = DIR == 0 ? 1 : 2, mv = DIR == 0 ? 2 : 1;
     const T pnL = DIR == 0 ? L.P : L.Q, ptL = DIR == 0 ? L.Q : L.P;
     const T pnR = DIR == 0 ? R.P : R.Q, ptR = DIR == 0 ? R.Q : R.P;
-    const T inv = T(1) / (L.s + R.s);
+    const T inv = frcp(L.s + R.s);
     const T u = (pnL + pnR) * inv;
     const T v = (ptL + ptR) * inv;
     const T H = (L.Hs + R.Hs) * inv;
     const T u2v2 = u * u + v * v;
     const T a2 = pr.gamma1 * (H - T(0.5) * u2v2);
-    const T a = sqrt(a2);
-    const T g1a2 = pr.gamma1 / a2;
+    const T ra = frsq(a2);
+    const T a = a2 * ra;
+    const T g1a2 = pr.gamma1 * (ra * ra);
     const T euv = H - u2v2;
-    const T i2a = T(1) / (T(2) * a);
+    const T i2a = T(0.5) * ra;
     const T d0 = R.q[0] - L.q[0], dn = R.q[mu] - L.q[mu], dt = R.q[mv] - L.q[mv],
             dE = R.q[3] - L.q[3];
     const T ae = g1a2 * (euv * d0 + u * dn + v * dt - dE);
@@ -217,36 +288,32 @@ template <class T> struct EulerP {
     const T a4 = (dn + (a - u) * d0 - a * ae) * i2a;
     const T a1 = d0 - ae - a4;
     const T s1 = u - a, s4 = u + a;
-    T sf1 = T(0), sfu = T(0), sf4 = T(0);
+    T sf1 = vmin(s1, T(0)), sfu = T(0), sf4 = T(0);
     const T s0 = pnL * L.rs - L.c;
-    if (!(s0 >= T(0) && s1 > T(0))) {
-      sf1 = s1 < T(0) ? s1 : T(0);
-      if (s0 < T(0)) {
-        const T r1 = L.q[0] + a1, n1 = L.q[mu] + a1 * s1, t1 = L.q[mv] + a1 * v,
-                E1 = L.q[3] + a1 * (H - u * a);
-        if (sonic_cross(r1, n1, t1, E1, pr, true)) {
-          const T s1r = sonic_speed(r1, n1, t1, E1, pr, T(-1));
-          sf1 = s0 * (s1r - s1) / (s1r - s0);
-        }
+    if (s0 < T(0)) {
+      const T r1 = L.q[0] + a1, n1 = L.q[mu] + a1 * s1, t1 = L.q[mv] + a1 * v,
+              E1 = L.q[3] + a1 * (H - u * a);
+      if (sonic_cross(r1, n1, t1, E1, pr, true)) {
+        const T s1r = sonic_speed(r1, n1, t1, E1, pr, T(-1));
+        sf1 = s0 * (s1r - s1) / (s1r - s0);
       }
-      if (u < T(0)) {
-        sfu = u;
-        sf4 = s4 < T(0) ? s4 : T(0);
+    }
+    if (u < T(0)) {
+      sfu = u;
+      sf4 = vmin(s4, T(0));
+      const T r4 = R.q[0] - a4, n4 = R.q[mu] - a4 * s4, t4 = R.q[mv] - a4 * v,
+              E4 = R.q[3] - a4 * (H + u * a);
+      if (sonic_cross(r4, n4, t4, E4, pr, false)) {
         const T s4r = pnR * R.rs + R.c;
-        const T r4 = R.q[0] - a4, n4 = R.q[mu] - a4 * s4, t4 = R.q[mv] - a4 * v,
-                E4 = R.q[3] - a4 * (H + u * a);
-        if (s4r > T(0) && sonic_cross(r4, n4, t4, E4, pr, false)) {
+        if (s4r > T(0)) {
           const T s4l = sonic_speed(r4, n4, t4, E4, pr, T(1));
           sf4 = s4l * (s4r - s4) / (s4r - s4l);
         }
       }
     }
     f.al[0] = a1; f.al[1] = ae; f.al[2] = as; f.al[3] = a4;
-    f.u = u; f.v = v; f.a = a; f.H = H; f.u2v2 = u2v2; f.g1a2 = g1a2; f.euv = euv; f.i2a = i2a;
-    T cm[4] = {sf1, sfu, sfu, sf4};
-    T cp[4] = {s1 - sf1, u - sfu, u - sfu, s4 - sf4};
-    combine(f, cm, f.am);
-    combine(f, cp, f.ap);
+    f.sf[0] = sf1; f.sf[1] = sfu; f.sf[2] = sfu; f.sf[3] = sf4;
+    f.u = u; f.v = v; f.a = a; f.H = H; f.i2a = i2a;
   }
 
   __device__ __forceinline__ static T smax(const Iface& f) { return (f.u < T(0) ? -f.u : f.u) + f.a; }
@@ -258,43 +325,55 @@ template <class T> struct EulerP {
     o[0] = f.al[0]; o[1] = f.al[1]; o[2] = f.al[2]; o[3] = f.al[3];
     o[4] = f.u; o[5] = f.v; o[6] = f.a; o[7] = f.H;
   }
-  __device__ __forceinline__ static void dots(const T A[NL], const T B[NL], T d[NW]) {
+  __device__ __forceinline__ static void mdot(const T A[NL], const T B[NL], T m[NW]) {
     const T uA = A[4], vA = A[5], aA = A[6], HA = A[7];
     const T uB = B[4], vB = B[5], aB = B[6], HB = B[7];
-    const T vv = vA * vB;
-    d[0] = (A[0] * B[0]) * (((T(1) + (uA - aA) * (uB - aB)) + vv) + (HA - uA * aA) * (HB - uB * aB));
-    d[1] = (A[1] * B[1]) * (((T(1) + uA * uB) + vv) +
-                            (T(0.5) * (uA * uA + vA * vA)) * (T(0.5) * (uB * uB + vB * vB)));
-    d[2] = (A[2] * B[2]) * (T(1) + vv);
-    d[3] = (A[3] * B[3]) * (((T(1) + (uA + aA) * (uB + aB)) + vv) + (HA + uA * aA) * (HB + uB * aB));
+    const T vv1 = fma(vA, vB, T(1));
+    m[0] = fma(HA - uA * aA, HB - uB * aB, fma(uA - aA, uB - aB, vv1));
+    m[1] = fma(T(0.25) * (uA * uA + vA * vA), uB * uB + vB * vB, fma(uA, uB, vv1));
+    m[2] = vv1;
+    m[3] = fma(HA + uA * aA, HB + uB * aB, fma(uA + aA, uB + aB, vv1));
   }
-  __device__ __forceinline__ static void combine(const Iface& f, const T coef[NW], T o[NEQ]) {
-    const T k1 = coef[0] * f.al[0], ke = coef[1] * f.al[1], ks = coef[2] * f.al[2],
-            k4 = coef[3] * f.al[3];
-    o[0] = k1 + ke + k4;
-    o[1] = k1 * (f.u - f.a) + ke * f.u + k4 * (f.u + f.a);
-    o[2] = (k1 + ke + k4) * f.v + ks;
-    o[3] = k1 * (f.H - f.u * f.a) + ke * (T(0.5) * f.u2v2) + ks * f.v + k4 * (f.H + f.u * f.a);
+  __device__ __forceinline__ static void inv_norm(const Iface& f, T in[NW]) {
+    const T vv1 = fma(f.v, f.v, T(1));
+    const T s1 = f.u - f.a, s4 = f.u + f.a, e1 = f.H - f.u * f.a, e4 = f.H + f.u * f.a;
+    const T k = T(0.5) * (f.u * f.u + f.v * f.v);
+    const T n1 = fma(e1, e1, fma(s1, s1, vv1));
+    const T n2 = fma(k, k, fma(f.u, f.u, vv1));
+    const T n4 = fma(e4, e4, fma(s4, s4, vv1));
+    const T n14 = n1 * n4, n23 = n2 * vv1;
+    const T r = frcp(n14 * n23);
+    const T r14 = n23 * r, r23 = n14 * r;
+    in[0] = n4 * r14;
+    in[1] = vv1 * r23;
+    in[2] = n2 * r23;
+    in[3] = n1 * r14;
   }
-  __device__ __forceinline__ static void rpt2(const Iface& f, const T as[NEQ], T bm[NEQ], T bp[NEQ]) {
-    const T a3 = f.g1a2 * (f.euv * as[0] + f.u * as[1] + f.v * as[2] - as[3]);
-    const T a2 = as[1] - f.u * as[0];
-    const T a4 = (as[2] + (f.a - f.v) * as[0] - f.a * a3) * f.i2a;
-    const T a1 = as[0] - a3 - a4;
-    const T sg1 = f.v - f.a, sg4 = f.v + f.a;
-    const T m1 = tmin(sg1, T(0)) * a1, me = tmin(f.v, T(0)) * a3, ms = tmin(f.v, T(0)) * a2,
-            m4 = tmin(sg4, T(0)) * a4;
-    const T q1 = tmax(sg1, T(0)) * a1, qe = tmax(f.v, T(0)) * a3, qs = tmax(f.v, T(0)) * a2,
-            q4 = tmax(sg4, T(0)) * a4;
-    const T hk = T(0.5) * f.u2v2;
-    bm[0] = m1 + me + m4;
-    bm[1] = (m1 + me + m4) * f.u + ms;
-    bm[2] = m1 * sg1 + me * f.v + m4 * sg4;
-    bm[3] = m1 * (f.H - f.v * f.a) + me * hk + ms * f.u + m4 * (f.H + f.v * f.a);
-    bp[0] = q1 + qe + q4;
-    bp[1] = (q1 + qe + q4) * f.u + qs;
-    bp[2] = q1 * sg1 + qe * f.v + q4 * sg4;
-    bp[3] = q1 * (f.H - f.v * f.a) + qe * hk + qs * f.u + q4 * (f.H + f.v * f.a);
+  __device__ __forceinline__ static void expand_n(const Iface& f, const T c[NW], T o[NEQ]) {
+    const T sum = c[0] + c[1] + c[3];
+    o[0] = sum;
+    o[1] = c[0] * (f.u - f.a) + c[1] * f.u + c[3] * (f.u + f.a);
+    o[2] = fma(sum, f.v, c[2]);
+    o[3] = c[0] * (f.H - f.u * f.a) + c[1] * (T(0.5) * (f.u * f.u + f.v * f.v)) + c[2] * f.v +
+           c[3] * (f.H + f.u * f.a);
+  }
+  __device__ __forceinline__ static void tbeta(const Iface& f, const T A[NW], T b[NW]) {
+    const T m = T(0.5) * (A[0] + A[3]), d = A[2] * f.i2a;
+    b[0] = m - d;
+    b[1] = A[1];
+    b[2] = f.a * (A[3] - A[0]);
+    b[3] = m + d;
+  }
+  __device__ __forceinline__ static void tspeeds(const Iface& f, T sg[NW]) {
+    sg[0] = f.v - f.a; sg[1] = f.v; sg[2] = f.v; sg[3] = f.v + f.a;
+  }
+  __device__ __forceinline__ static void expand_t(const Iface& f, const T c[NW], T o[NEQ]) {
+    const T sum = c[0] + c[1] + c[3];
+    o[0] = sum;
+    o[1] = fma(sum, f.u, c[2]);
+    o[2] = c[0] * (f.v - f.a) + c[1] * f.v + c[3] * (f.v + f.a);
+    o[3] = c[0] * (f.H - f.v * f.a) + c[1] * (T(0.5) * (f.u * f.u + f.v * f.v)) + c[2] * f.u +
+           c[3] * (f.H + f.v * f.a);
   }
 };
 

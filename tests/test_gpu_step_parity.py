@@ -77,6 +77,8 @@ def test_dam_break_100_steps():
     g.upload(q)
     g.advance(101)
     ctrl = g.sync_cur()
+    print("ctrl: numsteps %d attempts %d rejected %d status %d t %.17g" %
+          (ctrl.numsteps, ctrl.attempts, ctrl.rejected, ctrl.status, ctrl.t))
     assert ctrl.numsteps == 100 and ctrl.rejected == st["rejected"] == 1
     assert abs(ctrl.t - st["t"]) <= 1e-12 * st["t"]
     err = np.abs(g.download() - qo).max() / np.abs(q).max()
