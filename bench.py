@@ -31,7 +31,21 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-n", type=int, default=384)
     p.add_argument("--cpu-steps", type=int, default=40)
+    p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                   help="only run a plain device copy of BYTES bytes 5 times (calibrates the "
+                        "FETCH_SIZE/WRITE_SIZE counters) and exit")
     return p.parse_args()
+
+
+def copy_probe(nbytes):
+    import torch
+    n = nbytes // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "repeats": 5}), flush=True)
 
 
 def run_gpu(n, dtype_id, steps, warmup, seed):
@@ -89,6 +103,9 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.init()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     r64 = run_gpu(a.n, 0, a.steps, a.warmup, a.seed)
     ms_step = r64["ms"] / r64["accepted"]
     value = r64["cells"] * r64["accepted"] / (r64["ms"] * 1e-3)

@@ -408,7 +408,7 @@ __global__ void control_step(claw2d_ctrl* c, double dx, double dy, double cfl_ma
   // speed in an attempt that would be accepted ends the run with CLAW_STATUS_BAD_STATE
   const bool bad = (c->status & CLAW_STATUS_BAD_STATE) || cfl != cfl;
   c->status &= ~CLAW_STATUS_BAD_STATE;
-  if (bad && !(cfl > cfl_max && dtvar)) {
+  if (bad && !(cfl > cfl_max)) {
     c->status |= CLAW_STATUS_BAD_STATE;
     c->done = 1;
     return;

@@ -1,0 +1,199 @@
+"""ClawSolver2D with kernel_language='HIP' (upstream pyclaw/src/pyclaw/classic/solver.py).
+
+The unsplit classic step (step2/flux2/limiter with rpn2/rpt2) and the evolve_to_time loop run on
+the GPU through libclawpack_amd.so: one fused kernel per step attempt plus a device-side
+accept/reject/dt controller, so the host only synchronises at output times.  There is no CPU
+compute path: a missing library or device raises.
+"""
+import math
+
+import numpy as np
+
+from .. import _lib
+from ..device import DeviceGrid, make_desc
+from .limiters import tvd
+from .solver import BC, Solver
+
+_KERNEL_LANGUAGES = ("HIP",)
+
+
+class ClawSolver2D(Solver):
+    """Two-dimensional classic (wave-propagation) solver.
+
+    Supported on the HIP path: dimensional_split = False, order 1 or 2, transverse_waves 0/1/2,
+    limiters 0-5 per wave, BC extrap/periodic/wall, the Riemann plugins
+    shallow_roe_with_efix_2D and euler_4wave_2D, solver.precision 'fp64' (default) or 'fp32'.
+    """
+
+    def __init__(self, riemann_solver=None, claw_package=None):
+        super().__init__(riemann_solver, claw_package)
+        self.num_dim = 2
+        self.dimensional_split = True   # pyclaw's default; the HIP path needs False
+        self.transverse_waves = 1
+        self.order = 2
+        self.limiters = tvd.minmod
+        self.fwave = False
+        self.kernel_language = "HIP"
+        self.reflect_index = [1, 2]
+        self.precision = "fp64"
+        self.seg_h = 0
+        self._grid = None
+        self._desc_key = None
+
+    # ---------------------------------------------------------------------------- set-up ---
+    def _mthlim(self):
+        lim = self.limiters if isinstance(self.limiters, (list, tuple)) else [self.limiters]
+        if len(lim) == 1:
+            lim = list(lim) * self.num_waves
+        if len(lim) != self.num_waves:
+            raise Exception("Length of solver.limiters must be 1 or num_waves")
+        return [int(x) for x in lim]
+
+    def _check_supported(self, state):
+        if self.kernel_language not in _KERNEL_LANGUAGES:
+            raise NotImplementedError(
+                "kernel_language=%r is not available in clawpack_amd; the only compute path is "
+                "'HIP' (the reference's f2py Fortran/Python kernels are not part of this package)"
+                % (self.kernel_language,))
+        if self.rp is None or not hasattr(self.rp, "kernel_id"):
+            raise NotImplementedError("Riemann solver not available on the HIP path: use "
+                                      "riemann.shallow_roe_with_efix_2D or riemann.euler_4wave_2D")
+        if self.dimensional_split:
+            raise NotImplementedError("dimensional_split=True (step2ds) is not implemented on the HIP "
+                                      "path; set solver.dimensional_split = False (unsplit step2)")
+        if self.fwave:
+            raise NotImplementedError("fwave=True is not supported on the HIP path")
+        if state.num_aux > 0 or state.index_capa >= 0:
+            raise NotImplementedError("aux arrays / capacity are not supported on the HIP path")
+        if self.num_ghost != 2:
+            raise NotImplementedError("num_ghost must be 2")
+        if self.before_step is not None or self.step_source is not None:
+            raise NotImplementedError("before_step/step_source callbacks are not supported on the HIP "
+                                      "path (the time loop runs on the device)")
+        if self.order not in (1, 2) or self.transverse_waves not in (0, 1, 2):
+            raise ValueError("order must be 1 or 2 and transverse_waves 0, 1 or 2")
+        for k in range(2):
+            lo, hi = self.bc_lower[k], self.bc_upper[k]
+            if lo is None or hi is None:
+                raise Exception("Lower and upper boundary conditions must be specified for every "
+                                "dimension (solver.bc_lower / solver.bc_upper)")
+            if BC.custom in (lo, hi):
+                raise NotImplementedError("BC.custom (user_bc) is not supported on the HIP path")
+            if (lo == BC.periodic) != (hi == BC.periodic):
+                raise Exception("Periodic boundary conditions must be set on both sides")
+        if self.precision not in ("fp64", "fp32"):
+            raise ValueError("solver.precision must be 'fp64' or 'fp32'")
+        if state.num_eqn != self.rp.num_eqn:
+            raise ValueError("state.num_eqn does not match the Riemann solver")
+        for key in self.rp.problem_data_keys:
+            if key not in state.problem_data:
+                raise KeyError("state.problem_data[%r] must be set for %s" % (key, self.rp.__name__))
+
+    def _make_desc(self, state):
+        mx, my = state.grid.num_cells
+        dx, dy = state.grid.delta
+        pd = state.problem_data
+        return make_desc(
+            self.rp.kernel_id, _lib.FP64 if self.precision == "fp64" else _lib.FP32, mx, my, dx, dy,
+            self.rp.num_eqn, self.rp.num_waves, order=self.order,
+            transverse_waves=self.transverse_waves, mthlim=self._mthlim(),
+            bc_lower=self.bc_lower, bc_upper=self.bc_upper, reflect_index=self.reflect_index,
+            grav=pd.get("grav", 1.0), gamma=pd.get("gamma", 1.4), cfl_max=self.cfl_max,
+            cfl_desired=self.cfl_desired, dt_max=self.dt_max, dt_variable=self.dt_variable,
+            seg_h=self.seg_h)
+
+    def setup(self, solution):
+        state = solution.state
+        self._check_supported(state)
+        self._check_cfl_settings()
+        if self.dt is None:
+            self.dt = self.dt_initial
+        self._device(state)
+        self._is_set_up = True
+
+    def _device(self, state):
+        desc = self._make_desc(state)
+        key = (desc.mx, desc.my, desc.num_eqn, desc.dtype)
+        if self._grid is None or self._desc_key != key:
+            self._grid = DeviceGrid(desc)
+            self._desc_key = key
+        else:
+            self._grid.desc = desc
+        return self._grid
+
+    # ----------------------------------------------------------------------------- steps ---
+    def step_hyperbolic(self, solution):
+        """One step2 with the current dt (no accept/reject), as ClawSolver2D.step_hyperbolic."""
+        if not self._is_set_up:
+            self.setup(solution)
+        state = solution.state
+        self._check_supported(state)
+        g = self._device(state)
+        g.upload(state.q)
+        g.init_ctrl(solution.t, self.dt, target_steps=1)
+        cfl, c = g.step(self.dt)
+        state.q[...] = g.download()
+        self.cfl.update_global_max(cfl)
+        if c.status & _lib.STATUS_BAD_STATE:
+            raise FloatingPointError("HIP step produced a non-positive depth/density or NaN")
+        return cfl
+
+    def evolve_to_time(self, solution, tend=None):
+        """Advance solution to tend (or by one step attempt if tend is None), with pyclaw's
+        accept/reject and CFL-driven dt, on the device."""
+        if not self._is_set_up:
+            self.setup(solution)
+        state = solution.state
+        self._check_supported(state)
+        self._check_cfl_settings()
+        g = self._device(state)
+        tstart = solution.t
+        take_one_step = tend is None
+        self.status["cflmax"] = self.cfl.get_cached_max()
+        self.status["dtmin"] = self.dt
+        self.status["dtmax"] = self.dt
+        self.status["numsteps"] = 0
+        g.upload(state.q)
+        if take_one_step:
+            g.init_ctrl(tstart, self.dt, target_steps=1, max_steps=1)
+        else:
+            g.init_ctrl(tstart, self.dt, tend=tend, max_steps=self.max_steps)
+        c = g.read_ctrl()
+        attempts_left = self.max_steps if not take_one_step else 1
+        while not c.done and attempts_left > 0:
+            if take_one_step:
+                chunk = 1
+            else:
+                est = (tend - c.t) / c.dt if c.dt > 0 else 1.0
+                chunk = int(min(max(math.ceil(est) + 1, 1), 1024, attempts_left))
+            g.advance(chunk)
+            attempts_left -= chunk
+            c = g.read_ctrl()
+        g.sync_cur(c)
+        state.q[...] = g.download()
+        solution.t = c.t
+        self.cfl.update_global_max(c.cfl)
+        self.status["cflmax"] = max(self.status["cflmax"], c.cflmax)
+        self.status["dtmin"] = min(self.status["dtmin"], c.dtmin)
+        self.status["dtmax"] = max(self.status["dtmax"], c.dtmax)
+        self.status["numsteps"] += int(c.numsteps)
+        self.status["rejected"] = int(c.rejected)
+        if c.status & _lib.STATUS_BAD_STATE:
+            raise FloatingPointError("HIP step produced a non-positive depth/density or a NaN "
+                                     "wave speed at t=%g" % c.t)
+        if c.status & _lib.STATUS_CFL_FIXED_DT:
+            raise Exception("CFL too large, giving up!")
+        self.dt = c.dt
+        if (c.status & _lib.STATUS_MAX_STEPS) or (not take_one_step and not c.done):
+            raise Exception("Maximum number of timesteps have been taken")
+        return self.status
+
+    # ------------------------------------------------------------------------- extras ------
+    def total(self, solution, m=0):
+        """Compensated FP64 sum of q[m] over the grid, computed on the device in a fixed order
+        (for FP32 states this is the conservation check of the FP32 path)."""
+        if not self._is_set_up:
+            self.setup(solution)
+        g = self._device(solution.state)
+        g.upload(solution.state.q)
+        return g.mass_sum(m)

@@ -1,0 +1,83 @@
+"""C ABI of libclawpack_amd.so: loads without a GPU, exports every function declared in
+include/clawpack_amd.h, struct layouts match the binding, descriptor validation (no compute)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "clawpack_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(claw2d_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("claw2d_step", "claw2d_advance", "claw2d_validate", "claw2d_ctrl_init",
+                 "claw2d_mass_sum", "claw2d_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    raw = C.CDLL(_lib.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(raw, name), name
+        assert name in _lib.SIGNATURES, "binding lacks %s" % name
+    assert lib is not None
+
+
+def test_struct_layouts():
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    assert lib.claw2d_abi_struct_size(0) == C.sizeof(_lib.ClawDesc)
+    assert lib.claw2d_abi_struct_size(1) == C.sizeof(_lib.ClawCtrl)
+
+
+def _desc(**kw):
+    from clawpack_amd.device import make_desc
+    args = dict(solver_id=0, dtype=0, mx=32, my=16, dx=0.1, dy=0.1, num_eqn=3, num_waves=3)
+    args.update(kw)
+    return make_desc(**args)
+
+
+@pytest.mark.parametrize("bad,code", [
+    (dict(mx=1), -5), (dict(num_eqn=4), -3), (dict(solver_id=7), -2), (dict(dtype=3), -4),
+    (dict(num_ghost=3), -6), (dict(order=3), -7), (dict(transverse_waves=3), -8),
+    (dict(mthlim=(9, 4, 4)), -9), (dict(bc_lower=(0, 1)), -10), (dict(bc_lower=(2, 1)), -11),
+    (dict(reflect_index=(5, 2)), -12), (dict(dx=0.0), -13),
+])
+def test_validate_rejects(bad, code):
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    d = _desc(**bad)
+    assert lib.claw2d_validate(C.byref(d)) == code
+    assert lib.claw2d_last_error()
+
+
+def test_validate_accepts_and_sizes():
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    d = _desc()
+    assert lib.claw2d_validate(C.byref(d)) == 0
+    assert lib.claw2d_state_bytes(C.byref(d)) == 3 * 32 * 16 * 8
+    d32 = _desc(dtype=1, solver_id=1, num_eqn=4, num_waves=4)
+    assert lib.claw2d_state_bytes(C.byref(d32)) == 4 * 32 * 16 * 4
+
+
+def test_ctrl_init_clips_to_tend():
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    d = _desc()
+    c = _lib.ClawCtrl()
+    assert lib.claw2d_ctrl_init(C.byref(d), C.byref(c), 0.0, 0.1, 0.05, 1, 0, 0) == 0
+    assert c.dt == 0.05 and c.done == 0 and c.flags & _lib.FLAG_RUN_TO_TEND
+    assert lib.claw2d_ctrl_init(C.byref(d), C.byref(c), 0.0, 0.1, 0.0, 0, 5, 0) == 0
+    assert c.target_steps == 5 and c.dt == 0.1 and not (c.flags & _lib.FLAG_RUN_TO_TEND)
+    assert lib.claw2d_ctrl_init(C.byref(d), C.byref(c), 0.0, -1.0, 0.0, 0, 5, 0) == -21

@@ -1,0 +1,137 @@
+"""The PyClaw-compatible front end driving the HIP path end to end, against the oracle and the
+golden fixtures; error behaviour forced from the host."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from clawpack_amd import problems, pyclaw, riemann
+from oracle import corc
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+
+
+def setup_dam_break(n=200, precision="fp64"):
+    solver = pyclaw.ClawSolver2D(riemann.shallow_roe_with_efix_2D)
+    solver.dimensional_split = False
+    solver.transverse_waves = 2
+    solver.limiters = pyclaw.limiters.tvd.MC
+    solver.cfl_desired = 0.9
+    solver.precision = precision
+    solver.bc_lower = [pyclaw.BC.extrap, pyclaw.BC.extrap]
+    solver.bc_upper = [pyclaw.BC.extrap, pyclaw.BC.extrap]
+    x = pyclaw.Dimension(-2.5, 2.5, n, name="x")
+    y = pyclaw.Dimension(-2.5, 2.5, n, name="y")
+    domain = pyclaw.Domain([x, y])
+    state = pyclaw.State(domain, solver.num_eqn)
+    state.problem_data["grav"] = 1.0
+    X, Y = state.grid.p_centers
+    state.q[0] = np.where(np.sqrt(X ** 2 + Y ** 2) < 0.5, 2.0, 1.0)
+    state.q[1] = state.q[2] = 0.0
+    return solver, pyclaw.Solution(state, domain)
+
+
+def test_controller_dam_break_matches_oracle():
+    print("case controller dam break 200x200 tfinal 0.5", flush=True)
+    solver, sol = setup_dam_break()
+    q0 = sol.state.q.copy()
+    claw = pyclaw.Controller()
+    claw.solution, claw.solver = sol, solver
+    claw.tfinal, claw.num_output_times = 0.5, 5
+    claw.keep_copy = True
+    status = claw.run()
+    assert len(claw.frames) == 6 and abs(sol.t - 0.5) < 1e-15
+    # the oracle driven through the same output times
+    d = corc.make_desc("swe", 200, 200, 0.025, 0.025, grav=1.0)
+    c = corc.make_ctrl()
+    q, t, dt, n = q0, 0.0, 0.1, 0
+    for tout in np.linspace(0, 0.5, 6)[1:]:
+        q, st = corc.evolve(d, c, q, tend=tout, t0=t, dt=dt)
+        t, dt, n = st["t"], st["dt"], n + st["numsteps"]
+    assert status["numsteps"] == n
+    assert np.abs(sol.state.q - q).max() <= 1e-12 * 2.0
+    assert abs(solver.dt - dt) <= 1e-12 * dt
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[:-4] for p in FIXTURES])
+def test_golden_fixture_on_device(path):
+    from clawpack_amd.device import DeviceGrid, make_desc
+    name = os.path.basename(path)[:-4]
+    print("case golden", name, flush=True)
+    ref = np.load(path)
+    solver = str(ref["solver"])
+    q0 = ref["q0"]
+    neq, mx, my = q0.shape
+    bc = tuple(int(b) for b in ref["bc"])
+    d = make_desc(0 if solver == "swe" else 1, 0, mx, my, float(ref["dx"]), float(ref["dy"]), neq, neq,
+                  bc_lower=bc, bc_upper=bc, grav=1.0, gamma=1.4)
+    g = DeviceGrid(d)
+    g.upload(q0)
+    steps = int(ref["numsteps"])
+    g.init_ctrl(0.0, 0.1, target_steps=steps)
+    g.advance(steps + int(ref["rejected"]) + 2)
+    c = g.sync_cur()
+    assert c.numsteps == steps and c.rejected == int(ref["rejected"]) and c.status == 0
+    assert abs(c.t - float(ref["t"])) <= 1e-12 * float(ref["t"])
+    err = np.abs(g.download() - ref["q"]).max() / np.abs(q0).max()
+    assert err <= 1e-12, err
+
+
+def test_euler_through_front_end():
+    print("case euler quadrants 128x128 via pyclaw", flush=True)
+    q0, dx, dy = problems.euler_quadrants(128, 128, 1)
+    solver = pyclaw.ClawSolver2D(riemann.euler_4wave_2D)
+    solver.dimensional_split = False
+    solver.transverse_waves = 2
+    solver.limiters = pyclaw.limiters.tvd.MC
+    solver.bc_lower = [pyclaw.BC.extrap] * 2
+    solver.bc_upper = [pyclaw.BC.extrap] * 2
+    domain = pyclaw.Domain([pyclaw.Dimension(0, 1, 128, name="x"), pyclaw.Dimension(0, 1, 128, name="y")])
+    state = pyclaw.State(domain, 4)
+    state.problem_data["gamma"] = 1.4
+    state.q[...] = q0
+    sol = pyclaw.Solution(state, domain)
+    solver.evolve_to_time(sol, 0.1)
+    d = corc.make_desc("euler", 128, 128, dx, dy, gamma=1.4)
+    qo, st = corc.evolve(d, corc.make_ctrl(), q0, tend=0.1)
+    assert solver.status["numsteps"] == st["numsteps"]
+    assert np.abs(state.q - qo).max() <= 1e-12 * np.abs(q0).max()
+
+
+def test_take_one_step_and_step_hyperbolic():
+    solver, sol = setup_dam_break(64)
+    solver.setup(sol)
+    q0 = sol.state.q.copy()
+    solver.dt = 0.01
+    cfl = solver.step_hyperbolic(sol)
+    d = corc.make_desc("swe", 64, 64, 5 / 64, 5 / 64, grav=1.0)
+    qo, cflo = corc.step_hyperbolic(d, corc.make_ctrl(), q0, 0.01)
+    assert abs(cfl - cflo) <= 1e-14 and np.abs(sol.state.q - qo).max() <= 1e-13
+    st = solver.evolve_to_time(sol)  # tend=None: one attempt
+    assert st["numsteps"] == 1
+
+
+def test_bad_state_raises():
+    solver, sol = setup_dam_break(48)
+    sol.state.q[0, 10, 10] = -1.0  # negative depth forced from the host
+    with pytest.raises(FloatingPointError):
+        solver.evolve_to_time(sol, 0.2)
+
+
+def test_fixed_dt_cfl_violation_raises():
+    solver, sol = setup_dam_break(48)
+    solver.dt_variable = False
+    solver.dt = 0.5
+    with pytest.raises(Exception, match="CFL too large"):
+        solver.evolve_to_time(sol, 1.0)
+
+
+def test_max_steps_raises():
+    solver, sol = setup_dam_break(48)
+    solver.max_steps = 3
+    solver.dt = 0.01  # no rejected attempt: pyclaw raises only when max_steps steps were accepted
+    with pytest.raises(Exception, match="Maximum number of timesteps"):
+        solver.evolve_to_time(sol, 5.0)

@@ -1,0 +1,94 @@
+"""PyClaw-compatible front end on the CPU: containers, defaults and the refusal paths.
+Nothing here computes a step; the HIP path refuses to run without a device."""
+import numpy as np
+import pytest
+
+from clawpack_amd import pyclaw, riemann
+from conftest import gpu_available
+
+
+def make(nx=20, ny=10, solver_mod=None):
+    solver_mod = solver_mod or riemann.shallow_roe_with_efix_2D
+    solver = pyclaw.ClawSolver2D(solver_mod)
+    solver.dimensional_split = False
+    solver.bc_lower = [pyclaw.BC.extrap, pyclaw.BC.extrap]
+    solver.bc_upper = [pyclaw.BC.extrap, pyclaw.BC.extrap]
+    x = pyclaw.Dimension(-2.5, 2.5, nx, name="x")
+    y = pyclaw.Dimension(-2.5, 2.5, ny, name="y")
+    domain = pyclaw.Domain([x, y])
+    state = pyclaw.State(domain, solver.num_eqn)
+    state.problem_data["grav"] = 1.0
+    state.problem_data["gamma"] = 1.4
+    state.q[0] = 1.0
+    if solver.num_eqn == 4:
+        state.q[3] = 2.5
+    return solver, state, domain, pyclaw.Solution(state, domain)
+
+
+def test_geometry_and_state():
+    solver, state, domain, sol = make()
+    assert state.q.shape == (3, 20, 10)
+    assert domain.grid.x.delta == 0.25 and domain.grid.num_cells == [20, 10]
+    X, Y = state.grid.p_centers
+    assert X.shape == (20, 10) and X[0, 0] == -2.375 and Y[0, 1] == -1.75
+    assert sol.t == 0.0 and sol.state is state
+    d2 = pyclaw.Domain((0.0, 0.0), (1.0, 2.0), (4, 8))
+    assert d2.grid.delta == [0.25, 0.25]
+
+
+def test_defaults_follow_pyclaw():
+    s = pyclaw.ClawSolver2D(riemann.euler_4wave_2D)
+    assert (s.num_eqn, s.num_waves) == (4, 4)
+    assert s.dimensional_split is True and s.transverse_waves == 1 and s.order == 2
+    assert s.limiters == pyclaw.limiters.tvd.minmod
+    assert (s.cfl_max, s.cfl_desired, s.dt_initial, s.max_steps) == (1.0, 0.9, 0.1, 10000)
+    assert s.kernel_language == "HIP"
+
+
+@pytest.mark.parametrize("attr,value,exc", [
+    ("kernel_language", "Fortran", NotImplementedError),
+    ("kernel_language", "Python", NotImplementedError),
+    ("dimensional_split", True, NotImplementedError),
+    ("fwave", True, NotImplementedError),
+    ("order", 3, ValueError),
+    ("precision", "fp16", ValueError),
+])
+def test_unsupported_configurations_raise(attr, value, exc):
+    solver, state, domain, sol = make()
+    setattr(solver, attr, value)
+    with pytest.raises(exc):
+        solver.setup(sol)
+
+
+def test_bc_checks():
+    solver, state, domain, sol = make()
+    solver.bc_lower = [pyclaw.BC.custom, pyclaw.BC.extrap]
+    with pytest.raises(NotImplementedError):
+        solver.setup(sol)
+    solver.bc_lower = [pyclaw.BC.periodic, pyclaw.BC.extrap]
+    with pytest.raises(Exception, match="Periodic"):
+        solver.setup(sol)
+    solver.bc_lower = [None, pyclaw.BC.extrap]
+    with pytest.raises(Exception, match="boundary conditions"):
+        solver.setup(sol)
+
+
+def test_problem_data_required():
+    solver, state, domain, sol = make()
+    del state.problem_data["grav"]
+    with pytest.raises(KeyError):
+        solver.setup(sol)
+
+
+def test_limiter_list_length():
+    solver, state, domain, sol = make()
+    solver.limiters = [1, 4]
+    with pytest.raises(Exception, match="limiters"):
+        solver.setup(sol)
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device refusal")
+def test_no_cpu_fallback_without_device():
+    solver, state, domain, sol = make()
+    with pytest.raises(RuntimeError, match="HIP device"):
+        solver.evolve_to_time(sol, 0.1)
