@@ -31,6 +31,10 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-n", type=int, default=384)
     p.add_argument("--cpu-steps", type=int, default=40)
+    p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--batch-grids", type=int, default=512)
+    p.add_argument("--batch-steps", type=int, default=200)
+    p.add_argument("--seq-steps", type=int, default=10)
     p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES bytes 5 times (calibrates the "
                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -80,6 +84,47 @@ def run_gpu(n, dtype_id, steps, warmup, seed):
     assert c1.status == 0, "device status %d" % c1.status
     return dict(ms=ms, wall=wall, accepted=accepted, attempts=attempts, cells=n * n,
                 t=c1.t, dt=c1.dt, cfl=c1.cfl)
+
+
+def run_batch(ngrids, dtype_id, steps, seq_steps, seed):
+    """Batched workload: ngrids independent periodic hump grids (sides U{64..512}), each with its
+    own CFL-0.9 dt; `steps` attempts as one replayed HIP graph vs one-grid-at-a-time launches."""
+    import torch
+    from clawpack_amd import problems
+    from clawpack_amd.batch import BatchedGrids
+    from clawpack_amd.device import make_desc
+
+    grids = problems.batch_workload(ngrids, seed)
+    d = make_desc(0, dtype_id, 8, 8, 1.0, 1.0, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
+                  bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
+    out = {}
+    s = torch.cuda.current_stream()
+    for mode in ("graph", "sequential"):
+        b = BatchedGrids(d, grids)
+        b.init(0.0, 1e-3, target_steps=10 ** 9)
+        b.advance(3)  # warm-up (first attempts settle each grid's dt)
+        n_att = steps if mode == "graph" else seq_steps
+        h = b.make_graph(n_att) if mode == "graph" else None
+        torch.cuda.synchronize()
+        c0 = b.read_ctrls()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        if mode == "graph":
+            b.replay(h)
+        else:
+            b.advance_sequential(n_att)
+        e1.record(s)
+        torch.cuda.synchronize()
+        c1 = b.read_ctrls()
+        ms = e0.elapsed_time(e1)
+        upd = sum(b.shapes[g][0] * b.shapes[g][1] * (c1[g].numsteps - c0[g].numsteps)
+                  for g in range(b.n))
+        assert all(c.status == 0 for c in c1)
+        out[mode] = {"ms": ms, "attempts": n_att, "cell_updates_per_s": upd / (ms * 1e-3)}
+        del b
+    out["cells"] = sum(q.shape[1] * q.shape[2] for q, _, _ in grids)
+    out["speedup_vs_sequential"] = out["graph"]["cell_updates_per_s"] / out["sequential"]["cell_updates_per_s"]
+    return out
 
 
 def cpu_baseline(n, steps, seed):
@@ -139,6 +184,9 @@ def main():
         out["fp32"] = {"value": v32, "ms_per_step": r32["ms"] / r32["accepted"],
                        "ratio_vs_fp64": v32 / value,
                        "roofline_frac": 24.0 * v32 / 1e9 / HBM_PEAK_GBS}
+    if not a.no_batch:
+        out["batch"] = {"grids": a.batch_grids, "fp64": run_batch(a.batch_grids, 0, a.batch_steps, a.seq_steps, a.seed),
+                        "fp32": run_batch(a.batch_grids, 1, a.batch_steps, a.seq_steps, a.seed)}
     if not a.no_cpu:
         rate, el = cpu_baseline(a.cpu_n, a.cpu_steps, a.seed)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",

@@ -38,6 +38,14 @@ class ClawCtrl(C.Structure):
     ]
 
 
+class ClawBatchGrid(C.Structure):
+    _fields_ = [
+        ("offset", C.c_int64), ("mx", C.c_int32), ("my", C.c_int32), ("nstrips", C.c_int32),
+        ("nsegs", C.c_int32), ("seg_h", C.c_int32), ("item0", C.c_int32),
+        ("reserved", C.c_int32 * 2), ("dx", C.c_double), ("dy", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/clawpack_amd.h
 _VP = C.c_void_p
 _PD = C.POINTER(ClawDesc)
@@ -50,6 +58,14 @@ SIGNATURES = {
     "claw2d_advance": (C.c_int, [_PD, _VP, _VP, _VP, C.c_longlong, _VP]),
     "claw2d_mass_sum_workspace_bytes": (C.c_size_t, [_PD]),
     "claw2d_mass_sum": (C.c_int, [_PD, _VP, C.c_int32, _VP, _VP, _VP]),
+    "claw2d_batch_prepare": (C.c_int, [_PD, C.POINTER(ClawBatchGrid), C.c_int32, C.c_int32,
+                                       C.POINTER(C.c_int64)]),
+    "claw2d_batch_advance": (C.c_int, [_PD, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP,
+                                       C.c_longlong, _VP]),
+    "claw2d_batch_graph_create": (C.c_int, [_PD, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP,
+                                            C.c_longlong, C.POINTER(C.c_void_p)]),
+    "claw2d_graph_launch": (C.c_int, [_VP, _VP]),
+    "claw2d_graph_destroy": (C.c_int, [_VP]),
     "claw2d_abi_struct_size": (C.c_size_t, [C.c_int32]),
     "claw2d_last_error": (C.c_char_p, []),
 }
@@ -76,7 +92,8 @@ def load():
         fn.restype = res
         fn.argtypes = args
     if lib.claw2d_abi_struct_size(0) != C.sizeof(ClawDesc) or \
-            lib.claw2d_abi_struct_size(1) != C.sizeof(ClawCtrl):
+            lib.claw2d_abi_struct_size(1) != C.sizeof(ClawCtrl) or \
+            lib.claw2d_abi_struct_size(2) != C.sizeof(ClawBatchGrid):
         raise ImportError("claw2d_desc/claw2d_ctrl layout mismatch between the binding and the library")
     _lib = lib
     return lib

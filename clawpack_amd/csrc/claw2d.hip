@@ -183,29 +183,19 @@ template <int NEQ, class T> __device__ __forceinline__ void yswap(T v[NEQ]) {
   v[2] = t;
 }
 
+// One work item: the strip of columns [x0, x0+60) and rows [y0, y1) of one grid, advanced by dt
+// from qin to qout.  Shared by the single-grid and the batched kernel.  The CFL speeds and the
+// bad-state flag of the item are reduced over the wavefront and folded into *ctrl atomically.
 // LIM > 0: every wave uses limiter LIM (compile-time); LIM == 0: per-wave ids from a.mthlim.
 template <class T, class P, int LIM>
-__global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
+__device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __restrict__ qin,
+                                           T* __restrict__ qout, const int mx, const int my,
+                                           const int x0, const int y0, const int y1, const double dt,
+                                           const double dx, const double dy, claw2d_ctrl* ctrl) {
   constexpr int NEQ = P::NEQ, NW = P::NW, NC = P::NC, NL = P::NL;
   using Cell = typename P::Cell;
   using Iface = typename P::Iface;
-
-  double dt = a.dt_fixed;
-  const T* qin = a.q0;
-  T* qout = a.q1;
-  if (a.use_ctrl) {
-    if (a.ctrl->done) return;
-    dt = a.ctrl->dt;
-    if (a.ctrl->cur) { qin = a.q1; qout = a.q0; }
-  }
   const int lane = threadIdx.x & 63;
-  // wave-uniform quantities live in SGPRs (readfirstlane), so the row loop branches on SCC
-  const int wave = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6));
-  if (wave >= a.nstrips * a.nsegs) return;
-  const int strip = wave % a.nstrips, seg = wave / a.nstrips;
-  const int mx = a.mx, my = a.my;
-  const int x0 = strip * kOwn, y0 = seg * a.seg_h;
-  const int y1 = min(y0 + a.seg_h, my);
   const size_t plane = (size_t)mx * (size_t)my;
   const int c = x0 - 2 + lane;
   int cx;
@@ -216,7 +206,7 @@ __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
   const bool own_col = lane >= 2 && lane < 2 + kOwn && c < mx;
   const bool xcfl_col = lane >= 1 && c >= 0 && c <= mx;   // x-interfaces 1..mx+1 (1-based)
   const bool ycfl_col = c >= -1 && c <= mx;               // y-sweeps on columns 0..mx+1
-  const T dtdx = T(dt / a.dx), dtdy = T(dt / a.dy);
+  const T dtdx = T(dt / dx), dtdy = T(dt / dy);
 
   // Loads row R of this lane's column.  The wall sign flips are returned, not applied, so the
   // consumer (one row later) and not the load waits for the data.
@@ -365,12 +355,73 @@ __global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
     fyprev = fynew;
   }
 
+  // Wave-uniform epilogue: every lane issues the atomics with the same operand (the compiler
+  // folds them into one per wave).  A lane-0-only branch here lets the compiler peel lanes 1-63
+  // off into their own loop when this function runs inside the batched kernel's work loop.
   const T wx = wave_max(smx), wy = wave_max(smy);
-  const unsigned long long anybad = __ballot(bad);
-  if (lane == 0) {
-    atomic_max_nonneg(&a.ctrl->smax_bits[0], (double)wx);
-    atomic_max_nonneg(&a.ctrl->smax_bits[1], (double)wy);
-    if (anybad) atomicOr(&a.ctrl->status, (int)CLAW_STATUS_BAD_STATE);
+  const bool anybad = __ballot(bad) != 0ull;
+  atomic_max_nonneg(&ctrl->smax_bits[0], (double)wx);
+  atomic_max_nonneg(&ctrl->smax_bits[1], (double)wy);
+  if (anybad) atomicOr(&ctrl->status, (int)CLAW_STATUS_BAD_STATE);
+}
+
+template <class T, class P, int LIM>
+__global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
+  double dt = a.dt_fixed;
+  const T* qin = a.q0;
+  T* qout = a.q1;
+  if (a.use_ctrl) {
+    if (a.ctrl->done) return;
+    dt = a.ctrl->dt;
+    if (a.ctrl->cur) { qin = a.q1; qout = a.q0; }
+  }
+  // wave-uniform quantities live in SGPRs (readfirstlane), so the row loop branches on SCC
+  const int wave = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6));
+  if (wave >= a.nstrips * a.nsegs) return;
+  const int strip = wave % a.nstrips, seg = wave / a.nstrips;
+  const int x0 = strip * kOwn, y0 = seg * a.seg_h;
+  const int y1 = min(y0 + a.seg_h, a.my);
+  march_item<T, P, LIM>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
+}
+
+// Batched grids: every grid's planes live in one ping-pong pair at its own offset, each grid has
+// its own control block (its own CFL-chosen dt), and a persistent set of wavefronts takes
+// (grid, strip, segment) work items from an atomic counter.  The counter is reset by the batch
+// control kernel, so a captured sequence of launches contains no memset.
+struct BatchDev {
+  const claw2d_batch_grid* grids;
+  claw2d_ctrl* ctrls;
+  int* counter;
+  int ngrids;
+  int total_items;
+};
+
+template <class T, class P, int LIM>
+__global__ void __launch_bounds__(kLanes* kWPB) batch_step(StepArgs<T, P> a, BatchDev b) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    // every lane takes part (lane 0 adds 1, the others 0) and lane 0's ticket is broadcast, so
+    // no lane-divergent region sits inside this loop
+    const int item = __builtin_amdgcn_readfirstlane(atomicAdd(b.counter, lane == 0 ? 1 : 0));
+    if (item >= b.total_items) break;
+    int lo = 0, hi = b.ngrids - 1;  // last grid whose item0 <= item
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (b.grids[mid].item0 <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const int gi = __builtin_amdgcn_readfirstlane(lo);
+    const claw2d_batch_grid& G = b.grids[gi];
+    claw2d_ctrl* c = b.ctrls + gi;
+    if (c->done) continue;
+    const double dt = c->dt;
+    const T* qin = (c->cur ? a.q1 : a.q0) + G.offset;
+    T* qout = (c->cur ? a.q0 : a.q1) + G.offset;
+    const int local = item - G.item0;
+    const int strip = local % G.nstrips, seg = local / G.nstrips;
+    const int x0 = strip * kOwn, y0 = seg * G.seg_h;
+    const int y1 = min(y0 + G.seg_h, G.my);
+    march_item<T, P, LIM>(a, qin, qout, G.mx, G.my, x0, y0, y1, dt, G.dx, G.dy, c);
   }
 }
 
@@ -388,9 +439,8 @@ __global__ void finalize_step(claw2d_ctrl* c, double dt, double dx, double dy) {
 
 // One evolve_to_time iteration after the step (upstream pyclaw solver.py): accept/reject,
 // advance t, choose the next dt, clip it to tend.
-__global__ void control_step(claw2d_ctrl* c, double dx, double dy, double cfl_max,
-                             double cfl_desired, double dt_max) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void control_one(claw2d_ctrl* c, double dx, double dy, double cfl_max,
+                                            double cfl_desired, double dt_max) {
   if (c->done) return;
   const double sx = __longlong_as_double((long long)c->smax_bits[0]);
   const double sy = __longlong_as_double((long long)c->smax_bits[1]);
@@ -452,6 +502,21 @@ __global__ void control_step(claw2d_ctrl* c, double dx, double dy, double cfl_ma
     if (c->tend - c->t - ndt < 1.0e-14 * c->t) ndt = c->tend - c->t;
   }
   c->dt = ndt;
+}
+
+__global__ void control_step(claw2d_ctrl* c, double dx, double dy, double cfl_max,
+                             double cfl_desired, double dt_max) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  control_one(c, dx, dy, cfl_max, cfl_desired, dt_max);
+}
+
+// One thread per grid; thread 0 also rearms the work counter of the batched step.
+__global__ void batch_control(const claw2d_batch_grid* grids, claw2d_ctrl* ctrls, int* counter,
+                              int ngrids, double cfl_max, double cfl_desired, double dt_max) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0) *counter = 0;
+  if (g >= ngrids) return;
+  control_one(ctrls + g, grids[g].dx, grids[g].dy, cfl_max, cfl_desired, dt_max);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -626,6 +691,75 @@ static int dispatch_step(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* 
              : launch_step<float, EulerP<float>>(d, euler_par<float>(d), q0, q1, ctrl, use_ctrl, dt, s);
 }
 
+template <class T, class P>
+static int launch_batch(const claw2d_desc* d, const typename P::Par& par, const claw2d_batch_grid* grids,
+                        int ngrids, int total_items, void* q0, void* q1, claw2d_ctrl* ctrls,
+                        int32_t* counter, hipStream_t s) {
+  bool same = true;
+  for (int p = 1; p < P::NW; ++p) same = same && d->mthlim[p] == d->mthlim[0];
+  const int lim = same ? d->mthlim[0] : -1;
+  const void* kern = lim == 4 ? (const void*)batch_step<T, P, 4>
+                   : (lim == 1 ? (const void*)batch_step<T, P, 1> : (const void*)batch_step<T, P, 0>);
+  static thread_local long cached[3] = {-1, -1, -1};
+  const int slot = lim == 4 ? 0 : (lim == 1 ? 1 : 2);
+  if (cached[slot] < 0) cached[slot] = resident_waves(kern);
+  StepArgs<T, P> a;
+  fill_args(a, d, cached[slot]);
+  a.q0 = (T*)q0;
+  a.q1 = (T*)q1;
+  a.ctrl = nullptr;
+  a.use_ctrl = 0;
+  a.dt_fixed = 0.0;
+  a.par = par;
+  BatchDev b;
+  b.grids = grids;
+  b.ctrls = ctrls;
+  b.counter = counter;
+  b.ngrids = ngrids;
+  b.total_items = total_items;
+  const long res = cached[slot] > 0 ? cached[slot] : 4096;
+  const long waves = std::min<long>(total_items, res);
+  const int blocks = (int)((waves + kWPB - 1) / kWPB);
+  if (lim == 4)
+    hipLaunchKernelGGL((batch_step<T, P, 4>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a, b);
+  else if (lim == 1)
+    hipLaunchKernelGGL((batch_step<T, P, 1>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a, b);
+  else
+    hipLaunchKernelGGL((batch_step<T, P, 0>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a, b);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "batch_step launch");
+}
+
+static int batch_attempt(const claw2d_desc* d, const claw2d_batch_grid* grids, int ngrids,
+                         int total_items, void* q0, void* q1, claw2d_ctrl* ctrls, int32_t* counter,
+                         hipStream_t s) {
+  const bool swe = d->solver == CLAW_SOLVER_SHALLOW_ROE_EFIX;
+  int rc;
+  if (d->dtype == CLAW_FP64)
+    rc = swe ? launch_batch<double, SweP<double>>(d, swe_par<double>(d), grids, ngrids, total_items, q0, q1, ctrls, counter, s)
+             : launch_batch<double, EulerP<double>>(d, euler_par<double>(d), grids, ngrids, total_items, q0, q1, ctrls, counter, s);
+  else
+    rc = swe ? launch_batch<float, SweP<float>>(d, swe_par<float>(d), grids, ngrids, total_items, q0, q1, ctrls, counter, s)
+             : launch_batch<float, EulerP<float>>(d, euler_par<float>(d), grids, ngrids, total_items, q0, q1, ctrls, counter, s);
+  if (rc) return rc;
+  const int threads = 256, blocks = (ngrids + threads - 1) / threads;
+  hipLaunchKernelGGL(batch_control, dim3(blocks), dim3(threads), 0, s, grids, ctrls, (int*)counter,
+                     ngrids, d->cfl_max, d->cfl_desired, d->dt_max);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "batch_control launch");
+}
+
+static int check_batch(const claw2d_desc* d, const void* grids, int32_t ngrids, int64_t total_items,
+                       const void* q0, const void* q1, const void* ctrls, const void* counter) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!grids || !q0 || !q1 || !ctrls || !counter) return fail(-22, "null buffer");
+  if (q0 == q1) return fail(-23, "q0 and q1 must differ");
+  if (ngrids < 1) return fail(-30, "ngrids must be >= 1");
+  if (total_items < 1 || total_items > (1LL << 30)) return fail(-31, "total_items out of range");
+  return 0;
+}
+
 }  // namespace claw
 
 using namespace claw;
@@ -719,7 +853,85 @@ int claw2d_mass_sum(const claw2d_desc* d, const void* q, int32_t m, double* out,
 }
 
 size_t claw2d_abi_struct_size(int32_t which) {
-  return which == 0 ? sizeof(claw2d_desc) : (which == 1 ? sizeof(claw2d_ctrl) : 0);
+  return which == 0 ? sizeof(claw2d_desc)
+                    : (which == 1 ? sizeof(claw2d_ctrl) : (which == 2 ? sizeof(claw2d_batch_grid) : 0));
+}
+
+int claw2d_batch_prepare(const claw2d_desc* d, claw2d_batch_grid* grids, int32_t ngrids, int32_t seg_h,
+                         int64_t* total_items) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!grids || !total_items) return fail(-22, "null buffer");
+  if (ngrids < 1) return fail(-30, "ngrids must be >= 1");
+  if (seg_h < 0) return fail(-14, "seg_h must be >= 0");
+  long long items = 0;
+  for (int g = 0; g < ngrids; ++g) {
+    claw2d_batch_grid& G = grids[g];
+    if (G.mx < 2 || G.my < 2) return fail(-5, "every grid needs mx, my >= 2");
+    if (!(G.dx > 0) || !(G.dy > 0)) return fail(-13, "dx, dy must be positive");
+    if (G.offset < 0) return fail(-32, "negative grid offset");
+    const int h = seg_h > 0 ? seg_h : std::max(16, std::min(64, (int)G.my));
+    G.seg_h = h;
+    G.nstrips = (G.mx + kOwn - 1) / kOwn;
+    G.nsegs = (G.my + h - 1) / h;
+    G.item0 = (int32_t)items;
+    items += (long long)G.nstrips * G.nsegs;
+    if (items > (1LL << 30)) return fail(-31, "too many work items");
+  }
+  *total_items = items;
+  return 0;
+}
+
+int claw2d_batch_advance(const claw2d_desc* d, const claw2d_batch_grid* grids, int32_t ngrids,
+                         int64_t total_items, void* q0, void* q1, claw2d_ctrl* ctrls, int32_t* counter,
+                         long long attempts, void* stream) {
+  int rc = check_batch(d, grids, ngrids, total_items, q0, q1, ctrls, counter);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  for (long long k = 0; k < attempts; ++k) {
+    rc = batch_attempt(d, grids, ngrids, (int)total_items, q0, q1, ctrls, counter, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int claw2d_batch_graph_create(const claw2d_desc* d, const claw2d_batch_grid* grids, int32_t ngrids,
+                              int64_t total_items, void* q0, void* q1, claw2d_ctrl* ctrls,
+                              int32_t* counter, long long attempts, void** graph_exec_out) {
+  int rc = check_batch(d, grids, ngrids, total_items, q0, q1, ctrls, counter);
+  if (rc) return rc;
+  if (!graph_exec_out || attempts < 1) return fail(-33, "bad graph arguments");
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  hipGraph_t graph = nullptr;
+  e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) { hipStreamDestroy(s); return hip_fail(e, "hipStreamBeginCapture"); }
+  for (long long k = 0; k < attempts && rc == 0; ++k)
+    rc = batch_attempt(d, grids, ngrids, (int)total_items, q0, q1, ctrls, counter, s);
+  e = hipStreamEndCapture(s, &graph);
+  if (rc == 0 && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  if (rc == 0) {
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+  }
+  if (graph) hipGraphDestroy(graph);
+  hipStreamDestroy(s);
+  if (rc == 0) *graph_exec_out = (void*)exec;
+  return rc;
+}
+
+int claw2d_graph_launch(void* graph_exec, void* stream) {
+  if (!graph_exec) return fail(-22, "null graph");
+  const hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipGraphLaunch");
+}
+
+int claw2d_graph_destroy(void* graph_exec) {
+  if (!graph_exec) return 0;
+  const hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph_exec);
+  return e == hipSuccess ? 0 : hip_fail(e, "hipGraphExecDestroy");
 }
 
 const char* claw2d_last_error(void) { return g_err.c_str(); }

@@ -76,3 +76,16 @@ def random_euler(mx, my, seed, gamma=1.4):
     q[0], q[1], q[2] = rho, rho * u, rho * v
     q[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
     return q
+
+
+def batch_workload(ngrids, seed, nmin=64, nmax=512, lower=-2.5, upper=2.5):
+    """Independent grids with side lengths ~ U{nmin..nmax} (non-square allowed), each carrying
+    16-hump data on its own [lower, upper]^2 (dx = (upper-lower)/mx, dy = (upper-lower)/my);
+    grid g's humps use seed + 1 + g."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, size=(ngrids, 2))
+    out = []
+    for g, (mx, my) in enumerate(sizes):
+        q, dx, dy = gaussian_humps(int(mx), int(my), seed + 1 + g, lower=lower, upper=upper)
+        out.append((q, dx, dy))
+    return out

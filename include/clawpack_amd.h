@@ -116,7 +116,47 @@ size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc* d);
 int claw2d_mass_sum(const claw2d_desc* d, const void* q_dev, int32_t m, double* out_dev,
                     void* workspace_dev, void* stream);
 
-/* sizeof(claw2d_desc) for which = 0, sizeof(claw2d_ctrl) for which = 1 (binding checks). */
+/* ----------------------------------------------------------------------------------------
+ * Batched grids (no counterpart in the reference: pyclaw advances one grid per solver).
+ * All grids share one claw2d_desc (solver, dtype, method, BCs, physics; its mx/my/dx/dy are
+ * ignored) and one ping-pong pair of state buffers; grid g's planes start at element offset
+ * grids[g].offset in both buffers.  Each grid has its own claw2d_ctrl (own CFL-chosen dt,
+ * accept/reject, done flag).  One persistent kernel per attempt advances every grid.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct claw2d_batch_grid {
+  int64_t offset;           /* element offset of this grid's plane 0 (caller) */
+  int32_t mx, my;           /* interior cells (caller) */
+  int32_t nstrips, nsegs;   /* work decomposition (claw2d_batch_prepare) */
+  int32_t seg_h, item0;     /* rows per item, first work item (claw2d_batch_prepare) */
+  int32_t reserved[2];
+  double dx, dy;            /* (caller) */
+} claw2d_batch_grid;
+
+/* Fills the decomposition fields of grids[0..ngrids) (host memory) for rows-per-item seg_h
+ * (0 = library default) and returns the total number of work items. */
+int claw2d_batch_prepare(const claw2d_desc* d, claw2d_batch_grid* grids, int32_t ngrids,
+                         int32_t seg_h, int64_t* total_items);
+
+/* Enqueues `attempts` evolve_to_time attempts of every grid: one persistent batched step kernel
+ * and one per-grid control kernel per attempt.  grids_dev is the prepared table copied to the
+ * device; ctrls_dev holds ngrids control blocks; counter_dev is one int, zero before the first
+ * call (the control kernel rearms it). */
+int claw2d_batch_advance(const claw2d_desc* d, const claw2d_batch_grid* grids_dev, int32_t ngrids,
+                         int64_t total_items, void* q0_dev, void* q1_dev, claw2d_ctrl* ctrls_dev,
+                         int32_t* counter_dev, long long attempts, void* stream);
+
+/* Captures `attempts` batched attempts (arguments as claw2d_batch_advance) into a HIP graph and
+ * returns its executable instance; launch it with claw2d_graph_launch, free with
+ * claw2d_graph_destroy.  The captured sequence contains kernels only (no memset nodes). */
+int claw2d_batch_graph_create(const claw2d_desc* d, const claw2d_batch_grid* grids_dev,
+                              int32_t ngrids, int64_t total_items, void* q0_dev, void* q1_dev,
+                              claw2d_ctrl* ctrls_dev, int32_t* counter_dev, long long attempts,
+                              void** graph_exec_out);
+int claw2d_graph_launch(void* graph_exec, void* stream);
+int claw2d_graph_destroy(void* graph_exec);
+
+/* sizeof(claw2d_desc) for which = 0, sizeof(claw2d_ctrl) for which = 1,
+ * sizeof(claw2d_batch_grid) for which = 2 (binding checks). */
 size_t claw2d_abi_struct_size(int32_t which);
 
 /* Text of the last error on the calling thread ("" if none). */
