@@ -36,7 +36,8 @@ class ClawSolver2D(Solver):
         self.kernel_language = "HIP"
         self.reflect_index = [1, 2]
         self.precision = "fp64"
-        self.seg_h = 0
+        self.seg_h = 0          # rows per marching segment; 0 = occupancy-derived default
+        self.autotune = False   # True: seg_h from clawpack_amd.autotune (measured, cached)
         self._grid = None
         self._desc_key = None
 
@@ -113,6 +114,9 @@ class ClawSolver2D(Solver):
 
     def _device(self, state):
         desc = self._make_desc(state)
+        if self.autotune and self.seg_h == 0:
+            from .. import autotune
+            desc.seg_h = autotune.tuned_seg_h(desc)
         key = (desc.mx, desc.my, desc.num_eqn, desc.dtype)
         if self._grid is None or self._desc_key != key:
             self._grid = DeviceGrid(desc)

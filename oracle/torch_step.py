@@ -98,7 +98,8 @@ def rpn2_swe(ql, qr, grav):
     h1 = ql[0] + W1[0]
     hu1 = ql[1] + W1[1]
     s1r = hu1 / h1 - torch.sqrt(grav * torch.where(h1 > 0, h1, torch.ones_like(h1)))
-    t1 = (s0 < 0) & (s1r > 0)
+    # a Roe middle depth h1 <= 0 makes the upstream sqrt NaN, so its transonic test is false
+    t1 = (s0 < 0) & (h1 > 0) & (s1r > 0)
     sf1 = torch.where(t1, s0 * (s1r - s1) / _safe(s1r - s0, t1),
                       torch.where(s1 < 0, s1, torch.zeros_like(s1)))
     left2 = s2 < 0
@@ -106,7 +107,7 @@ def rpn2_swe(ql, qr, grav):
     h3 = qr[0] - W3[0]
     hu3 = qr[1] - W3[1]
     s3l = hu3 / h3 + torch.sqrt(grav * torch.where(h3 > 0, h3, torch.ones_like(h3)))
-    t3 = (s3l < 0) & (s3r > 0)
+    t3 = (h3 > 0) & (s3l < 0) & (s3r > 0)
     sf3 = torch.where(t3, s3l * (s3r - s3) / _safe(s3r - s3l, t3),
                       torch.where(s3 < 0, s3, torch.zeros_like(s3)))
     amdq = sf1 * W1
@@ -172,6 +173,7 @@ def rpn2_euler(ql, qr, gamma):
     supersonic = (s0 >= 0) & (s1 > 0)
     q1 = ql + W1
     s1r = q1[1] / q1[0] - _sound(q1, gamma)
+    # a middle state with negative pressure gives a NaN sound speed: the transonic test is false
     t1 = (s0 < 0) & (s1r > 0)
     sf1 = torch.where(t1, s0 * (s1r - s1) / _safe(s1r - s0, t1),
                       torch.where(s1 < 0, s1, torch.zeros_like(s1)))
