@@ -20,6 +20,19 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
+def profiled_traffic(n):
+    """HBM bytes per launch of the FP64 step kernel from the committed rocprofv3 summary (FETCH_SIZE
+    and WRITE_SIZE scaled by a copy-kernel calibration), if one exists for this size."""
+    path = os.path.join(ROOT, "profiles", "step2_fused_swe_fp64_%dx%d.md" % (n, n))
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -35,10 +48,18 @@ def parse():
     p.add_argument("--batch-grids", type=int, default=512)
     p.add_argument("--batch-steps", type=int, default=200)
     p.add_argument("--seq-steps", type=int, default=10)
+    p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--adjoint-n", type=int, default=512)
+    p.add_argument("--adjoint-steps", type=int, default=400)
+    p.add_argument("--adjoint-fd", type=int, default=4, help="finite-difference directions checked")
     p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES bytes 5 times (calibrates the "
                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     return p.parse_args()
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def copy_probe(nbytes):
@@ -127,6 +148,71 @@ def run_batch(ngrids, dtype_id, steps, seq_steps, seed):
     return out
 
 
+def run_adjoint(n, nsteps, seed, fd_dirs):
+    """Inversion workload: SWE n x n FP64 periodic, nsteps frozen-dt steps from 1 + eta0, h sampled
+    at 64 gauges every 10 steps, checkpoints every 20.  Times the forward (steps + checkpoint
+    stores + samples) and the backward (segment recompute + reverse steps) and checks the
+    gradient against central differences (eps 1e-6) along fd_dirs seeded unit directions."""
+    import numpy as np
+    import torch
+    from clawpack_amd import problems
+    from clawpack_amd.adjoint import cfl_dt_sequence, evolve_frozen
+    from clawpack_amd.device import make_desc
+
+    eta_true, gauges, dx, dy = problems.inversion_workload(n, n, seed=4)
+    d = make_desc(0, 0, n, n, dx, dy, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
+                  bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
+    q_true = torch.tensor(np.stack([1.0 + eta_true, 0 * eta_true, 0 * eta_true]), device="cuda")
+    dts = cfl_dt_sequence(d, q_true, nsteps)
+    g = torch.tensor(gauges)
+    with torch.no_grad():
+        obs = evolve_frozen(q_true, d, dts, g, 10, 20)[0][:, 0, :]
+    rng = np.random.default_rng(seed)
+    eta0 = 0.5 * eta_true + 1e-3 * rng.standard_normal(eta_true.shape)
+
+    def loss_of(eta):
+        z = torch.zeros_like(eta)
+        samples, _ = evolve_frozen(torch.stack([1.0 + eta, z, z]), d, dts, g, 10, 20)
+        return ((samples[:, 0, :] - obs) ** 2).sum()
+
+    fwd, bwd = [], []
+    for rep in range(4):
+        eta = torch.tensor(eta0, device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = loss_of(eta)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if rep:
+            fwd.append(t1 - t0)
+            bwd.append(t2 - t1)
+    grad = eta.grad.cpu().numpy()
+    gnorm = float(np.linalg.norm(grad))
+    rel_dir, rel_norm = [], []
+    eps = 1e-6
+    for k in range(fd_dirs):
+        v = rng.standard_normal(eta0.shape)
+        v /= np.linalg.norm(v)
+        with torch.no_grad():
+            lp = float(loss_of(torch.tensor(eta0 + eps * v, device="cuda")))
+            lm = float(loss_of(torch.tensor(eta0 - eps * v, device="cuda")))
+        fd = (lp - lm) / (2 * eps)
+        ad = float((grad * v).sum())
+        rel_dir.append(abs(fd - ad) / abs(ad))
+        rel_norm.append(abs(fd - ad) / gnorm)
+    f, b = float(np.median(fwd)), float(np.median(bwd))
+    return {"workload": "swe_inversion_periodic_mc_%dx%d_%dsteps_64gauges_fp64" % (n, n, nsteps),
+            "forward_ms": f * 1e3, "backward_ms": b * 1e3, "backward_over_forward": b / f,
+            "total_over_forward": (f + b) / f, "checkpoint_every": 20, "loss": float(loss),
+            "fd": {"eps": eps, "directions": fd_dirs,
+                   "max_abs_err_over_grad_norm": max(rel_norm) if rel_norm else None,
+                   "median_rel_err_vs_directional": float(np.median(rel_dir)) if rel_dir else None,
+                   "max_rel_err_vs_directional": max(rel_dir) if rel_dir else None}}
+
+
 def cpu_baseline(n, steps, seed):
     from clawpack_amd import problems
     from oracle import corc
@@ -152,6 +238,7 @@ def main():
         copy_probe(a.copy_probe)
         return
     r64 = run_gpu(a.n, 0, a.steps, a.warmup, a.seed)
+    progress("fp64 step: %.4f ms/attempt" % (r64["ms"] / r64["attempts"]))
     ms_step = r64["ms"] / r64["accepted"]
     value = r64["cells"] * r64["accepted"] / (r64["ms"] * 1e-3)
     bytes_per_cell = 48.0
@@ -174,19 +261,24 @@ def main():
                    "cells": r64["cells"], "accepted_steps": r64["accepted"],
                    "attempts": r64["attempts"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "step2_fused<double,SweP<double>> + control_step per attempt",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(a.n),
+                     "kernel": "step2_fused<double,SweP<double>,4> (one launch per attempt)",
                      "bytes_per_cell": bytes_per_cell},
     }
     if not a.no_fp32:
         r32 = run_gpu(a.n, 1, a.steps, a.warmup, a.seed)
         v32 = r32["cells"] * r32["accepted"] / (r32["ms"] * 1e-3)
+        progress("fp32 step: %.4f ms/attempt" % (r32["ms"] / r32["attempts"]))
         out["fp32"] = {"value": v32, "ms_per_step": r32["ms"] / r32["accepted"],
                        "ratio_vs_fp64": v32 / value,
                        "roofline_frac": 24.0 * v32 / 1e9 / HBM_PEAK_GBS}
     if not a.no_batch:
         out["batch"] = {"grids": a.batch_grids, "fp64": run_batch(a.batch_grids, 0, a.batch_steps, a.seq_steps, a.seed),
                         "fp32": run_batch(a.batch_grids, 1, a.batch_steps, a.seq_steps, a.seed)}
+        progress("batch done")
+    if not a.no_adjoint:
+        out["adjoint"] = run_adjoint(a.adjoint_n, a.adjoint_steps, a.seed, a.adjoint_fd)
+        progress("adjoint done")
     if not a.no_cpu:
         rate, el = cpu_baseline(a.cpu_n, a.cpu_steps, a.seed)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",

@@ -35,6 +35,7 @@ class ClawCtrl(C.Structure):
         ("numsteps", C.c_longlong), ("attempts", C.c_longlong), ("rejected", C.c_longlong),
         ("target_steps", C.c_longlong), ("max_steps", C.c_longlong),
         ("cur", C.c_int32), ("done", C.c_int32), ("status", C.c_int32), ("flags", C.c_int32),
+        ("arrive", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -58,6 +59,8 @@ SIGNATURES = {
     "claw2d_advance": (C.c_int, [_PD, _VP, _VP, _VP, C.c_longlong, _VP]),
     "claw2d_mass_sum_workspace_bytes": (C.c_size_t, [_PD]),
     "claw2d_mass_sum": (C.c_int, [_PD, _VP, C.c_int32, _VP, _VP, _VP]),
+    "claw2d_adjoint_workspace_bytes": (C.c_size_t, [_PD]),
+    "claw2d_adjoint_step": (C.c_int, [_PD, _VP, _VP, _VP, _VP, C.c_double, _VP]),
     "claw2d_batch_prepare": (C.c_int, [_PD, C.POINTER(ClawBatchGrid), C.c_int32, C.c_int32,
                                        C.POINTER(C.c_int64)]),
     "claw2d_batch_advance": (C.c_int, [_PD, _VP, C.c_int32, C.c_int64, _VP, _VP, _VP, _VP,

@@ -5,8 +5,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libclawpack_amd.so")
-SOURCES = ["claw2d.hip"]
-HEADERS = ["physics.hpp"]
+SOURCES = ["claw2d.hip", "adjoint.hip"]
+HEADERS = ["physics.hpp", "adjoint.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -1,0 +1,299 @@
+// Adjoint of the fused 2D unsplit shallow-water step (FP64, dt fixed) for gfx950, and its C ABI.
+//
+// lam_out = (d q^{n+1} / d q^n)^T lam_in, with q^{n+1} the step of claw2d_step from q^n.  The
+// step is a sum of per-interface contributions (see adjoint.hpp), so its transpose is a sum of
+// per-interface reverses, scattered onto the four cells each interface reads.  Kernels:
+//   adj_sweep<0>  one wavefront per (row line, segment of 58 columns).  Lane = column = the
+//                 x-interface on its left.  Each lane computes its interface's Roe split once; the
+//                 limiter partners' data and, in reverse, their adjoints travel one lane left or
+//                 right over ds_bpermute; the cell adjoint of column c is the sum of the shares of
+//                 interfaces c and c+1.  Writes the x-sweep share of every extended cell
+//                 (interior + 2 ghost layers) to `ext`, a (my+4) x (mx+4) workspace.
+//   adj_sweep<1>  the same along columns (lane = row position, strided loads), adding the
+//                 y-sweep share into `ext`.
+//   adj_fold      lam_out = lam_in + ext(interior) + the ghost-cell shares of ext folded back to
+//                 the interior cells they were copied from (extrap / periodic / wall, with the
+//                 wall's sign), in a fixed order (deterministic, no atomics).
+// Branches (entropy fix, limiter cases, upwind side, transverse speed signs) are taken as the
+// forward took them at q^n.  No counterpart upstream: PyClaw has no adjoint.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/clawpack_amd.h"
+#include "adjoint.hpp"
+
+namespace claw {
+int set_error(int code, const char* msg);
+int set_hip_error(hipError_t e, const char* where);
+
+namespace adj {
+
+constexpr int kOwnA = 58;  // owned positions per wavefront (3 halo lanes left, 3 right)
+constexpr int kWPBA = 4;   // wavefronts per workgroup
+
+__device__ __forceinline__ double bperm(double v, int addr) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ LimD bperm(const LimD& d, int addr) {
+  LimD o;
+  o.a1 = bperm(d.a1, addr); o.a2 = bperm(d.a2, addr); o.a3 = bperm(d.a3, addr);
+  o.s1 = bperm(d.s1, addr); o.s3 = bperm(d.s3, addr); o.v = bperm(d.v, addr);
+  return o;
+}
+
+struct AdjArgs {
+  const double* q;
+  const double* lam;
+  double* ext;
+  double* lam_out;
+  int mx, my;
+  double dtdx, dtdy;
+  Method M;
+  int bxl, bxh, byl, byh, rfx, rfy;
+  int nseg_x, nseg_y;
+};
+
+// state at (x, y) of the extended grid, ghost cells by index mapping, as frame (h, hn, ht)
+template <int DIR>
+__device__ __forceinline__ void load_q(const AdjArgs& a, int x, int y, double o[3]) {
+  bool fx, fy;
+  const int ix = bc_src(x, a.mx, a.bxl, a.bxh, fx);
+  const int iy = bc_src(y, a.my, a.byl, a.byh, fy);
+  const size_t plane = (size_t)a.mx * (size_t)a.my;
+  const size_t off = (size_t)iy * (size_t)a.mx + (size_t)ix;
+  double c[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double v = a.q[(size_t)m * plane + off];
+    c[m] = ((fx && m == a.rfx) != (fy && m == a.rfy)) ? -v : v;
+  }
+  o[0] = c[0];
+  o[1] = DIR == 0 ? c[1] : c[2];
+  o[2] = DIR == 0 ? c[2] : c[1];
+}
+
+// lambda at (x, y) as frame; zero outside the interior (ghost cells receive no update)
+template <int DIR>
+__device__ __forceinline__ void load_lam(const AdjArgs& a, int x, int y, double o[3]) {
+  if (x < 0 || x >= a.mx || y < 0 || y >= a.my) {
+    o[0] = o[1] = o[2] = 0.0;
+    return;
+  }
+  const size_t plane = (size_t)a.mx * (size_t)a.my;
+  const size_t off = (size_t)y * (size_t)a.mx + (size_t)x;
+  const double c0 = a.lam[off], c1 = a.lam[plane + off], c2 = a.lam[2 * plane + off];
+  o[0] = c0;
+  o[1] = DIR == 0 ? c1 : c2;
+  o[2] = DIR == 0 ? c2 : c1;
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(64 * kWPBA) adj_sweep(AdjArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * kWPBA + (threadIdx.x >> 6));
+  const int n_al = DIR == 0 ? a.mx : a.my;  // cells along the sweep line
+  const int n_ln = DIR == 0 ? a.my : a.mx;  // sweep lines
+  const int nseg = DIR == 0 ? a.nseg_x : a.nseg_y;
+  if (wave >= (n_ln + 4) * nseg) return;
+  const int li = wave / nseg;
+  const int line = li - 2;                            // -2 .. n_ln + 1
+  const int p = (wave - li * nseg) * kOwnA - 5 + lane;  // position along: lanes 3..60 own
+  const bool own = lane >= 3 && lane < 3 + kOwnA && p <= n_al + 1;
+  const int ew = a.mx + 4;
+  const size_t eplane = (size_t)ew * (size_t)(a.my + 4);
+  const size_t eoff = DIR == 0 ? (size_t)(line + 2) * ew + (size_t)(p + 2) : (size_t)(p + 2) * ew + (size_t)(line + 2);
+  // the forward sweeps lines -1 .. n_ln only
+  if (line < -1 || line > n_ln) {
+    if (DIR == 0 && own) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) a.ext[(size_t)m * eplane + eoff] = 0.0;
+    }
+    return;
+  }
+  const int aL = ((lane + 63) & 63) * 4, aR = ((lane + 1) & 63) * 4;
+  const double r = DIR == 0 ? a.dtdx : a.dtdy, rt = DIR == 0 ? a.dtdy : a.dtdx;
+
+  // ---- forward: cells p-1 | p, the Roe split and entropy fix of interface p
+  double qo[3];
+  if (DIR == 0) load_q<0>(a, p, line, qo); else load_q<1>(a, line, p, qo);
+  double ql[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) ql[m] = bperm(qo[m], aL);
+  const CellF cR = cell_fwd(qo[0], qo[1], qo[2]);
+  const CellF cL = cell_fwd(ql[0], ql[1], ql[2]);
+  const WaveF w = wave_fwd(cL, cR, a.M.g);
+  const FixF fx = fix_fwd(cL, cR, w, a.M.g, a.M.sqg);
+  const LimD me = lim_of(w);
+  const LimD nl = bperm(me, aL), nr = bperm(me, aR);
+
+  // ---- adjoints of the six output cells: (p-1 | p) x (line-1, line, line+1)
+  double r0[3], rm[3], rp[3];
+  if (DIR == 0) {
+    load_lam<0>(a, p, line, r0); load_lam<0>(a, p, line - 1, rm); load_lam<0>(a, p, line + 1, rp);
+  } else {
+    load_lam<1>(a, line, p, r0); load_lam<1>(a, line - 1, p, rm); load_lam<1>(a, line + 1, p, rp);
+  }
+  double bLX[3], bLG1[3], bLG2[3], bRX[3], bRG1[3], bRG2[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double l0 = bperm(r0[m], aL), lm = bperm(rm[m], aL), lp = bperm(rp[m], aL);
+    bLX[m] = l0;
+    bLG1[m] = l0 - lm;
+    bLG2[m] = lp - l0;
+    bRX[m] = r0[m];
+    bRG1[m] = r0[m] - rm[m];
+    bRG2[m] = rp[m] - r0[m];
+  }
+
+  // ---- reverse
+  WaveB Wb;
+  zero(Wb);
+  LimD NLb, NRb;
+  zero(NLb);
+  zero(NRb);
+  CellB Lb, Rb;
+  zero(Lb);
+  zero(Rb);
+  iface_core(cL, cR, w, fx, nl, nr, bLX, bLG1, bLG2, bRX, bRG1, bRG2, r, rt, a.M, Wb, NLb, NRb, Lb, Rb);
+  // this interface is the right partner of interface p-1 and the left partner of p+1
+  add_lim(Wb, bperm(NRb, aL));
+  add_lim(Wb, bperm(NLb, aR));
+  wave_rev(cL, cR, w, a.M.g, Wb, Lb, Rb);
+  // cell p: right cell of interface p, left cell of interface p+1
+  CellB tot = Rb;
+  tot.h += bperm(Lb.h, aR);
+  tot.hn += bperm(Lb.hn, aR);
+  tot.ht += bperm(Lb.ht, aR);
+  tot.s += bperm(Lb.s, aR);
+  tot.r += bperm(Lb.r, aR);
+  tot.P += bperm(Lb.P, aR);
+  tot.Q += bperm(Lb.Q, aR);
+  double o[3];
+  cell_rev(cR, tot, o);
+  if (own) {
+    const double c[3] = {o[0], DIR == 0 ? o[1] : o[2], DIR == 0 ? o[2] : o[1]};
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double* e = a.ext + (size_t)m * eplane + eoff;
+      if (DIR == 0) *e = c[m]; else *e += c[m];
+    }
+  }
+}
+
+// extended index k (0..4 = i, -2, -1, n, n+1) along one axis: does its value come from cell i, and
+// with the wall's sign flip?  (k = 0 is cell i itself.)
+__device__ __forceinline__ bool source_k(int k, int i, int n, int lo, int hi, int& e, bool& fl) {
+  e = k == 0 ? i : (k == 1 ? -2 : (k == 2 ? -1 : (k == 3 ? n : n + 1)));
+  fl = false;
+  if (k == 0) return true;
+  return bc_src(e, n, lo, hi, fl) == i;
+}
+
+__global__ void __launch_bounds__(256) adj_fold(AdjArgs a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long cells = (long long)a.mx * a.my;
+  if (t >= cells) return;
+  const int j = (int)(t / a.mx), i = (int)(t - (long long)j * a.mx);
+  const int ew = a.mx + 4;
+  const size_t eplane = (size_t)ew * (size_t)(a.my + 4);
+  const size_t plane = (size_t)cells;
+  double acc[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) acc[m] = a.lam[(size_t)m * plane + t];
+  if (i >= 2 && i < a.mx - 2 && j >= 2 && j < a.my - 2) {
+    const size_t eo = (size_t)(j + 2) * ew + (size_t)(i + 2);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) acc[m] += a.ext[(size_t)m * eplane + eo];
+  } else {
+    // fixed order: y candidate outer, x candidate inner
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+      int ey;
+      bool fy;
+      if (!source_k(b, j, a.my, a.byl, a.byh, ey, fy)) continue;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        int ex;
+        bool fx;
+        if (!source_k(c, i, a.mx, a.bxl, a.bxh, ex, fx)) continue;
+        const size_t eo = (size_t)(ey + 2) * ew + (size_t)(ex + 2);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          const double v = a.ext[(size_t)m * eplane + eo];
+          acc[m] += ((fx && m == a.rfx) != (fy && m == a.rfy)) ? -v : v;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) a.lam_out[(size_t)m * plane + t] = acc[m];
+}
+
+}  // namespace adj
+}  // namespace claw
+
+using namespace claw;
+using namespace claw::adj;
+
+static int check_adjoint_desc(const claw2d_desc* d) {
+  const int rc = claw2d_validate(d);
+  if (rc != 0) return rc;
+  if (d->solver != CLAW_SOLVER_SHALLOW_ROE_EFIX)
+    return set_error(-20, "the adjoint step is implemented for shallow water (shallow_roe_with_efix) only");
+  if (d->dtype != CLAW_FP64) return set_error(-21, "the adjoint step runs in FP64 only");
+  return 0;
+}
+
+extern "C" {
+
+size_t claw2d_adjoint_workspace_bytes(const claw2d_desc* d) {
+  if (check_adjoint_desc(d) != 0) return 0;
+  return (size_t)3 * (size_t)(d->mx + 4) * (size_t)(d->my + 4) * sizeof(double);
+}
+
+int claw2d_adjoint_step(const claw2d_desc* d, const void* q, const void* lam_in, void* lam_out, void* workspace,
+                        double dt, void* stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  const int rc = check_adjoint_desc(d);
+  if (rc != 0) return rc;
+  if (!q || !lam_in || !lam_out || !workspace) return set_error(-1, "null device pointer");
+  if (!(dt > 0.0)) return set_error(-22, "dt must be positive");
+  AdjArgs a;
+  a.q = (const double*)q;
+  a.lam = (const double*)lam_in;
+  a.ext = (double*)workspace;
+  a.lam_out = (double*)lam_out;
+  a.mx = d->mx;
+  a.my = d->my;
+  a.dtdx = dt / d->dx;
+  a.dtdy = dt / d->dy;
+  a.M.g = d->grav;
+  a.M.sqg = sqrt(d->grav);
+  a.M.order = d->order;
+  a.M.tw = d->transverse_waves;
+  for (int p = 0; p < 3; ++p) a.M.mthlim[p] = d->mthlim[p];
+  a.bxl = d->bc_lower[0];
+  a.bxh = d->bc_upper[0];
+  a.byl = d->bc_lower[1];
+  a.byh = d->bc_upper[1];
+  a.rfx = d->reflect_index[0];
+  a.rfy = d->reflect_index[1];
+  a.nseg_x = (d->mx + 4 + kOwnA - 1) / kOwnA;
+  a.nseg_y = (d->my + 4 + kOwnA - 1) / kOwnA;
+  const long long wx = (long long)(d->my + 4) * a.nseg_x, wy = (long long)(d->mx + 4) * a.nseg_y;
+  const long long cells = (long long)d->mx * d->my;
+  if (wx > (1LL << 30) || wy > (1LL << 30)) return set_error(-23, "grid too large for the adjoint launch");
+  hipLaunchKernelGGL(adj_sweep<0>, dim3((unsigned)((wx + kWPBA - 1) / kWPBA)), dim3(64 * kWPBA), 0, stream, a);
+  hipLaunchKernelGGL(adj_sweep<1>, dim3((unsigned)((wy + kWPBA - 1) / kWPBA)), dim3(64 * kWPBA), 0, stream, a);
+  hipLaunchKernelGGL(adj_fold, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, stream, a);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_hip_error(e, "adjoint launch");
+  return 0;
+}
+
+}  // extern "C"

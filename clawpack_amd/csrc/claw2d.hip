@@ -45,6 +45,9 @@ static int hip_fail(hipError_t e, const char* where) {
   g_err = std::string(where) + ": " + hipGetErrorString(e);
   return (int)e;
 }
+// error reporting for the library's other translation units (claw2d_last_error reads g_err)
+int set_error(int code, const char* msg) { return fail(code, msg); }
+int set_hip_error(hipError_t e, const char* where) { return hip_fail(e, where); }
 
 // ------------------------------------------------------------------------------------------
 // cross-lane helpers (LDS crossbar, not VALU)
@@ -111,6 +114,11 @@ template <class T, class P> struct StepArgs {
   int bcx_lo, bcx_hi, bcy_lo, bcy_hi, rfx, rfy;
   int order, tw;
   int mthlim[4];
+  // 0: no controller; 1: the last wavefront writes the CFL number of the fixed-dt step;
+  // 2: the last wavefront runs the evolve_to_time controller on ctrl
+  int fuse;
+  int total_waves;
+  double cfl_max, cfl_desired, dt_max;
   typename P::Par par;
 };
 
@@ -365,71 +373,20 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   if (anybad) atomicOr(&ctrl->status, (int)CLAW_STATUS_BAD_STATE);
 }
 
-template <class T, class P, int LIM>
-__global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
-  double dt = a.dt_fixed;
-  const T* qin = a.q0;
-  T* qout = a.q1;
-  if (a.use_ctrl) {
-    if (a.ctrl->done) return;
-    dt = a.ctrl->dt;
-    if (a.ctrl->cur) { qin = a.q1; qout = a.q0; }
-  }
-  // wave-uniform quantities live in SGPRs (readfirstlane), so the row loop branches on SCC
-  const int wave = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6));
-  if (wave >= a.nstrips * a.nsegs) return;
-  const int strip = wave % a.nstrips, seg = wave / a.nstrips;
-  const int x0 = strip * kOwn, y0 = seg * a.seg_h;
-  const int y1 = min(y0 + a.seg_h, a.my);
-  march_item<T, P, LIM>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
+// The speed maxima and the status bits were written by atomics of the step's wavefronts; read
+// them with device-scope atomic loads (a plain load may hit a stale line in this CU's L1 when
+// the controller runs inside the step kernel).
+__device__ __forceinline__ double smax_read(claw2d_ctrl* c, int k) {
+  return __longlong_as_double(
+      (long long)__hip_atomic_load(&c->smax_bits[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
-
-// Batched grids: every grid's planes live in one ping-pong pair at its own offset, each grid has
-// its own control block (its own CFL-chosen dt), and a persistent set of wavefronts takes
-// (grid, strip, segment) work items from an atomic counter.  The counter is reset by the batch
-// control kernel, so a captured sequence of launches contains no memset.
-struct BatchDev {
-  const claw2d_batch_grid* grids;
-  claw2d_ctrl* ctrls;
-  int* counter;
-  int ngrids;
-  int total_items;
-};
-
-template <class T, class P, int LIM>
-__global__ void __launch_bounds__(kLanes* kWPB) batch_step(StepArgs<T, P> a, BatchDev b) {
-  const int lane = threadIdx.x & 63;
-  for (;;) {
-    // every lane takes part (lane 0 adds 1, the others 0) and lane 0's ticket is broadcast, so
-    // no lane-divergent region sits inside this loop
-    const int item = __builtin_amdgcn_readfirstlane(atomicAdd(b.counter, lane == 0 ? 1 : 0));
-    if (item >= b.total_items) break;
-    int lo = 0, hi = b.ngrids - 1;  // last grid whose item0 <= item
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (b.grids[mid].item0 <= item) lo = mid;
-      else hi = mid - 1;
-    }
-    const int gi = __builtin_amdgcn_readfirstlane(lo);
-    const claw2d_batch_grid& G = b.grids[gi];
-    claw2d_ctrl* c = b.ctrls + gi;
-    if (c->done) continue;
-    const double dt = c->dt;
-    const T* qin = (c->cur ? a.q1 : a.q0) + G.offset;
-    T* qout = (c->cur ? a.q0 : a.q1) + G.offset;
-    const int local = item - G.item0;
-    const int strip = local % G.nstrips, seg = local / G.nstrips;
-    const int x0 = strip * kOwn, y0 = seg * G.seg_h;
-    const int y1 = min(y0 + G.seg_h, G.my);
-    march_item<T, P, LIM>(a, qin, qout, G.mx, G.my, x0, y0, y1, dt, G.dx, G.dy, c);
-  }
+__device__ __forceinline__ int status_read(claw2d_ctrl* c) {
+  return __hip_atomic_load(&c->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // CFL of a fixed-dt step (classic step2 returns cfl = max over both sweeps of dt/dx*|s|).
-__global__ void finalize_step(claw2d_ctrl* c, double dt, double dx, double dy) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double sx = __longlong_as_double((long long)c->smax_bits[0]);
-  const double sy = __longlong_as_double((long long)c->smax_bits[1]);
+__device__ __forceinline__ void finalize_one(claw2d_ctrl* c, double dt, double dx, double dy) {
+  const double sx = smax_read(c, 0), sy = smax_read(c, 1);
   c->smax_last[0] = sx;
   c->smax_last[1] = sy;
   c->smax_bits[0] = 0ull;
@@ -437,13 +394,10 @@ __global__ void finalize_step(claw2d_ctrl* c, double dt, double dx, double dy) {
   c->cfl = fmax(fmax(0.0, (dt / dx) * sx), (dt / dy) * sy);
 }
 
-// One evolve_to_time iteration after the step (upstream pyclaw solver.py): accept/reject,
-// advance t, choose the next dt, clip it to tend.
 __device__ __forceinline__ void control_one(claw2d_ctrl* c, double dx, double dy, double cfl_max,
                                             double cfl_desired, double dt_max) {
   if (c->done) return;
-  const double sx = __longlong_as_double((long long)c->smax_bits[0]);
-  const double sy = __longlong_as_double((long long)c->smax_bits[1]);
+  const double sx = smax_read(c, 0), sy = smax_read(c, 1);
   c->smax_last[0] = sx;
   c->smax_last[1] = sy;
   c->smax_bits[0] = 0ull;
@@ -456,8 +410,9 @@ __device__ __forceinline__ void control_one(claw2d_ctrl* c, double dx, double dy
   c->cfl = cfl;
   // a rejected attempt is discarded whatever it produced; a non-positive depth/density or a NaN
   // speed in an attempt that would be accepted ends the run with CLAW_STATUS_BAD_STATE
-  const bool bad = (c->status & CLAW_STATUS_BAD_STATE) || cfl != cfl;
-  c->status &= ~CLAW_STATUS_BAD_STATE;
+  const int status = status_read(c);
+  const bool bad = (status & CLAW_STATUS_BAD_STATE) || cfl != cfl;
+  c->status = status & ~CLAW_STATUS_BAD_STATE;
   if (bad && !(cfl > cfl_max)) {
     c->status |= CLAW_STATUS_BAD_STATE;
     c->done = 1;
@@ -504,11 +459,87 @@ __device__ __forceinline__ void control_one(claw2d_ctrl* c, double dx, double dy
   c->dt = ndt;
 }
 
-__global__ void control_step(claw2d_ctrl* c, double dx, double dy, double cfl_max,
-                             double cfl_desired, double dt_max) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  control_one(c, dx, dy, cfl_max, cfl_desired, dt_max);
+template <class T, class P, int LIM>
+__global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
+  double dt = a.dt_fixed;
+  const T* qin = a.q0;
+  T* qout = a.q1;
+  if (a.use_ctrl) {
+    if (a.ctrl->done) return;
+    dt = a.ctrl->dt;
+    if (a.ctrl->cur) { qin = a.q1; qout = a.q0; }
+  }
+  // wave-uniform quantities live in SGPRs (readfirstlane), so the row loop branches on SCC
+  const int wave = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6));
+  if (wave >= a.nstrips * a.nsegs) return;
+  const int strip = wave % a.nstrips, seg = wave / a.nstrips;
+  const int x0 = strip * kOwn, y0 = seg * a.seg_h;
+  const int y1 = min(y0 + a.seg_h, a.my);
+  march_item<T, P, LIM>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
+  if (a.fuse) {
+    // The last wavefront to finish closes the step: its speed maxima are then complete.  The
+    // wait for this wavefront's outstanding memory operations (its max/or atomics are acknowledged
+    // once performed device-wide) comes before its arrival; the controller reads the maxima with
+    // device-scope atomic loads.  No cache writeback is needed: the controller reads nothing else
+    // that this launch wrote.
+    const int lane = threadIdx.x & 63;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const int ticket = __builtin_amdgcn_readfirstlane(atomicAdd(&a.ctrl->arrive, lane == 0 ? 1 : 0));
+    if (ticket == a.total_waves - 1) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      if (lane == 0) {
+        if (a.fuse == 1) finalize_one(a.ctrl, dt, a.dx, a.dy);
+        else control_one(a.ctrl, a.dx, a.dy, a.cfl_max, a.cfl_desired, a.dt_max);
+        a.ctrl->arrive = 0;
+      }
+    }
+  }
 }
+
+// Batched grids: every grid's planes live in one ping-pong pair at its own offset, each grid has
+// its own control block (its own CFL-chosen dt), and a persistent set of wavefronts takes
+// (grid, strip, segment) work items from an atomic counter.  The counter is reset by the batch
+// control kernel, so a captured sequence of launches contains no memset.
+struct BatchDev {
+  const claw2d_batch_grid* grids;
+  claw2d_ctrl* ctrls;
+  int* counter;
+  int ngrids;
+  int total_items;
+};
+
+template <class T, class P, int LIM>
+__global__ void __launch_bounds__(kLanes* kWPB) batch_step(StepArgs<T, P> a, BatchDev b) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    // every lane takes part (lane 0 adds 1, the others 0) and lane 0's ticket is broadcast, so
+    // no lane-divergent region sits inside this loop
+    const int item = __builtin_amdgcn_readfirstlane(atomicAdd(b.counter, lane == 0 ? 1 : 0));
+    if (item >= b.total_items) break;
+    int lo = 0, hi = b.ngrids - 1;  // last grid whose item0 <= item
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (b.grids[mid].item0 <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const int gi = __builtin_amdgcn_readfirstlane(lo);
+    const claw2d_batch_grid& G = b.grids[gi];
+    claw2d_ctrl* c = b.ctrls + gi;
+    if (c->done) continue;
+    const double dt = c->dt;
+    const T* qin = (c->cur ? a.q1 : a.q0) + G.offset;
+    T* qout = (c->cur ? a.q0 : a.q1) + G.offset;
+    const int local = item - G.item0;
+    const int strip = local % G.nstrips, seg = local / G.nstrips;
+    const int x0 = strip * kOwn, y0 = seg * G.seg_h;
+    const int y1 = min(y0 + G.seg_h, G.my);
+    march_item<T, P, LIM>(a, qin, qout, G.mx, G.my, x0, y0, y1, dt, G.dx, G.dy, c);
+  }
+}
+
+
+// One evolve_to_time iteration after the step (upstream pyclaw solver.py): accept/reject,
+// advance t, choose the next dt, clip it to tend.
 
 // One thread per grid; thread 0 also rearms the work counter of the batched step.
 __global__ void batch_control(const claw2d_batch_grid* grids, claw2d_ctrl* ctrls, int* counter,
@@ -668,7 +699,12 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   a.use_ctrl = use_ctrl;
   a.dt_fixed = dt;
   a.par = par;
+  a.fuse = use_ctrl ? 2 : 1;
+  a.cfl_max = d->cfl_max;
+  a.cfl_desired = d->cfl_desired;
+  a.dt_max = d->dt_max;
   const int waves = a.nstrips * a.nsegs;
+  a.total_waves = waves;
   const int blocks = (waves + kWPB - 1) / kWPB;
   if (lim == 4)
     hipLaunchKernelGGL((step2_fused<T, P, 4>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
@@ -710,6 +746,9 @@ static int launch_batch(const claw2d_desc* d, const typename P::Par& par, const 
   a.ctrl = nullptr;
   a.use_ctrl = 0;
   a.dt_fixed = 0.0;
+  a.fuse = 0;
+  a.total_waves = 0;
+  a.cfl_max = a.cfl_desired = a.dt_max = 0.0;
   a.par = par;
   BatchDev b;
   b.grids = grids;
@@ -807,11 +846,7 @@ int claw2d_step(const claw2d_desc* d, const void* q_in, void* q_out, double dt, 
   if (q_in == q_out) return fail(-23, "q_in and q_out must differ");
   if (!(dt > 0)) return fail(-21, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
-  rc = dispatch_step(d, const_cast<void*>(q_in), q_out, ctrl, 0, dt, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(finalize_step, dim3(1), dim3(64), 0, s, ctrl, dt, d->dx, d->dy);
-  const hipError_t e = hipGetLastError();
-  return e == hipSuccess ? 0 : hip_fail(e, "finalize_step launch");
+  return dispatch_step(d, const_cast<void*>(q_in), q_out, ctrl, 0, dt, s);
 }
 
 int claw2d_advance(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, long long attempts,
@@ -824,10 +859,6 @@ int claw2d_advance(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, 
   for (long long k = 0; k < attempts; ++k) {
     rc = dispatch_step(d, q0, q1, ctrl, 1, 0.0, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(control_step, dim3(1), dim3(64), 0, s, ctrl, d->dx, d->dy, d->cfl_max,
-                       d->cfl_desired, d->dt_max);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "control_step launch");
   }
   return 0;
 }
@@ -906,7 +937,10 @@ int claw2d_batch_graph_create(const claw2d_desc* d, const claw2d_batch_grid* gri
   if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
   hipGraph_t graph = nullptr;
   e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) { hipStreamDestroy(s); return hip_fail(e, "hipStreamBeginCapture"); }
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(s);
+    return hip_fail(e, "hipStreamBeginCapture");
+  }
   for (long long k = 0; k < attempts && rc == 0; ++k)
     rc = batch_attempt(d, grids, ngrids, (int)total_items, q0, q1, ctrls, counter, s);
   e = hipStreamEndCapture(s, &graph);
@@ -916,8 +950,8 @@ int claw2d_batch_graph_create(const claw2d_desc* d, const claw2d_batch_grid* gri
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
   }
-  if (graph) hipGraphDestroy(graph);
-  hipStreamDestroy(s);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipStreamDestroy(s);
   if (rc == 0) *graph_exec_out = (void*)exec;
   return rc;
 }

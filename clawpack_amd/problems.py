@@ -89,3 +89,14 @@ def batch_workload(ngrids, seed, nmin=64, nmax=512, lower=-2.5, upper=2.5):
         q, dx, dy = gaussian_humps(int(mx), int(my), seed + 1 + g, lower=lower, upper=upper)
         out.append((q, dx, dy))
     return out
+
+
+def inversion_workload(mx=512, my=512, seed=4, n_humps=8, n_gauges=64, lower=-2.5, upper=2.5):
+    """Initial-surface inversion: true eta0 = 8 seeded Gaussian humps (the 16-hump recipe with
+    n_humps=8, base 0), gauges = n_gauges distinct seeded cells.  Returns (eta_true as (my, mx)
+    planes, gauge flat indices y*mx + x, dx, dy)."""
+    q, dx, dy = gaussian_humps(mx, my, seed, n_humps=n_humps, lower=lower, upper=upper, base=0.0)
+    eta = np.ascontiguousarray(q[0].T)
+    rng = np.random.default_rng(seed + 1000)
+    gauges = np.sort(rng.choice(mx * my, size=n_gauges, replace=False))
+    return eta, gauges, dx, dy

@@ -77,6 +77,8 @@ typedef struct claw2d_ctrl {
   int32_t done;
   int32_t status;                  /* CLAW_STATUS_* bits */
   int32_t flags;                   /* CLAW_FLAG_* bits */
+  int32_t arrive;                  /* wavefronts of the running step that have finished; 0 between calls */
+  int32_t reserved;
 } claw2d_ctrl;
 
 /* Checks a descriptor; 0 if the HIP path supports it. */
@@ -100,10 +102,11 @@ int claw2d_step(const claw2d_desc* d, const void* q_in_dev, void* q_out_dev, dou
                 claw2d_ctrl* ctrl_dev, void* stream);
 
 /* Enqueues `attempts` step attempts of Solver.evolve_to_time (upstream pyclaw solver.py) on the
- * ping-pong pair (q0_dev, q1_dev): each attempt runs one fused step with ctrl->dt from
- * q[ctrl->cur] into the other buffer, then one control kernel applies accept/reject
+ * ping-pong pair (q0_dev, q1_dev): each attempt is one kernel launch that steps with ctrl->dt
+ * from q[ctrl->cur] into the other buffer; its last wavefront to finish applies accept/reject
  * (cfl <= cfl_max), t += dt, cur ^= 1 on accept, and dt <- min(dt_max, dt*cfl_desired/cfl).
- * Once ctrl->done is set the remaining attempts are no-ops.  No host synchronisation. */
+ * Once ctrl->done is set the remaining attempts are no-ops.  No host synchronisation.  Calls
+ * that share one control block must not run concurrently. */
 int claw2d_advance(const claw2d_desc* d, void* q0_dev, void* q1_dev, claw2d_ctrl* ctrl_dev,
                    long long attempts, void* stream);
 
@@ -115,6 +118,21 @@ size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc* d);
  * counterpart in the reference (pyclaw sums with numpy in FP64). */
 int claw2d_mass_sum(const claw2d_desc* d, const void* q_dev, int32_t m, double* out_dev,
                     void* workspace_dev, void* stream);
+
+/* ----------------------------------------------------------------------------------------
+ * Adjoint (no counterpart in the reference: PyClaw has no adjoint).  Shallow water, FP64.
+ * ---------------------------------------------------------------------------------------- */
+
+/* Bytes of the workspace claw2d_adjoint_step needs: 3*(mx+4)*(my+4) doubles. */
+size_t claw2d_adjoint_workspace_bytes(const claw2d_desc* d);
+
+/* Reverse of one fixed-dt step: lam_out = (d q^{n+1} / d q^n)^T lam_in, where q^{n+1} is what
+ * claw2d_step makes from q_dev = q^n with this dt (same method, limiters, BCs, physics).  All
+ * arrays are SoA planes (3, my, mx) of doubles in device memory; lam_out may alias lam_in.
+ * Branches (entropy fix, limiter case, upwind side, transverse speed signs) are those the forward
+ * takes at q^n.  Three kernels on `stream`; the workspace is overwritten. */
+int claw2d_adjoint_step(const claw2d_desc* d, const void* q_dev, const void* lam_in_dev,
+                        void* lam_out_dev, void* workspace_dev, double dt, void* stream);
 
 /* ----------------------------------------------------------------------------------------
  * Batched grids (no counterpart in the reference: pyclaw advances one grid per solver).

@@ -81,3 +81,17 @@ def test_ctrl_init_clips_to_tend():
     assert lib.claw2d_ctrl_init(C.byref(d), C.byref(c), 0.0, 0.1, 0.0, 0, 5, 0) == 0
     assert c.target_steps == 5 and c.dt == 0.1 and not (c.flags & _lib.FLAG_RUN_TO_TEND)
     assert lib.claw2d_ctrl_init(C.byref(d), C.byref(c), 0.0, -1.0, 0.0, 0, 5, 0) == -21
+
+
+def test_adjoint_workspace_and_validation():
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    d = _desc()
+    assert lib.claw2d_adjoint_workspace_bytes(C.byref(d)) == 3 * (32 + 4) * (16 + 4) * 8
+    # Euler and FP32 have no adjoint: size 0 and a message
+    for bad in (_desc(solver_id=1, num_eqn=4, num_waves=4), _desc(dtype=1)):
+        assert lib.claw2d_adjoint_workspace_bytes(C.byref(bad)) == 0
+        assert b"adjoint" in lib.claw2d_last_error()
+        assert lib.claw2d_adjoint_step(C.byref(bad), None, None, None, None, 0.1, None) < 0
+    # null buffers and a non-positive dt are refused before any launch
+    assert lib.claw2d_adjoint_step(C.byref(d), None, None, None, None, 0.1, None) < 0
