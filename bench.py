@@ -42,12 +42,20 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-n", type=int, default=384)
-    p.add_argument("--cpu-steps", type=int, default=40)
+    p.add_argument("--cpu-n", type=int, default=640)
+    p.add_argument("--cpu-steps", type=int, default=100)
+    p.add_argument("--cpu-mt-n", type=int, default=1024)
+    p.add_argument("--cpu-mt-steps", type=int, default=200)
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--batch-grids", type=int, default=512)
     p.add_argument("--batch-steps", type=int, default=200)
     p.add_argument("--seq-steps", type=int, default=10)
+    p.add_argument("--no-euler", action="store_true")
+    p.add_argument("--euler-n", type=int, default=2048)
+    p.add_argument("--euler-steps", type=int, default=500)
+    p.add_argument("--no-precision", action="store_true")
+    p.add_argument("--precision-sizes", default="4096,8192")
+    p.add_argument("--precision-steps", type=int, default=1000)
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--adjoint-n", type=int, default=512)
     p.add_argument("--adjoint-steps", type=int, default=400)
@@ -148,6 +156,91 @@ def run_batch(ngrids, dtype_id, steps, seq_steps, seed):
     return out
 
 
+def run_euler(n, steps, warmup, seed):
+    """Euler gas dynamics: four-quadrant Riemann problem (seeded), euler_4wave with entropy fix,
+    unsplit, MC, extrapolation BCs, n x n on [0,1]^2, CFL 0.9 device-controlled dt."""
+    import torch
+    from clawpack_amd import problems
+    from clawpack_amd.device import DeviceGrid, make_desc
+
+    out = {}
+    q, dx, dy = problems.euler_quadrants(n, n, seed)
+    for dtype_id, name in ((0, "fp64"), (1, "fp32")):
+        d = make_desc(1, dtype_id, n, n, dx, dy, 4, 4, order=2, transverse_waves=2, mthlim=(4, 4, 4, 4),
+                      bc_lower=(1, 1), bc_upper=(1, 1), gamma=1.4)
+        g = DeviceGrid(d, "cuda:0")
+        g.upload(q)
+        g.init_ctrl(0.0, 1e-5, target_steps=10 ** 9)
+        g.advance(warmup)
+        torch.cuda.synchronize()
+        c0 = g.read_ctrl()
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        g.advance(steps)
+        e1.record(s)
+        torch.cuda.synchronize()
+        c1 = g.read_ctrl()
+        assert c1.status == 0, "device status %d" % c1.status
+        acc = int(c1.numsteps - c0.numsteps)
+        att = int(c1.attempts - c0.attempts)
+        ms = e0.elapsed_time(e1)
+        bpc = 64.0 if dtype_id == 0 else 32.0
+        out[name] = {"cell_updates_per_s": n * n * acc / (ms * 1e-3), "ms_per_step": ms / acc,
+                     "accepted": acc, "attempts": att, "t": c1.t,
+                     "roofline_frac": bpc * n * n * att / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        del g
+    out["workload"] = "euler_4wave_efix_unsplit_mc_extrap_quadrants_seed%d_%dx%d" % (seed, n, n)
+    return out
+
+
+def run_precision(sizes, steps, seed):
+    """FP32 beside FP64 on the periodic 16-hump shallow-water workload, same fixed dt sequence
+    (CFL about 0.8 of the initial wave speed), compensated FP64 mass sum after every step.
+    Reports L1 / Linf of the FP32 state against the FP64 state per component (the FP64 HIP path,
+    parity-proven against the oracle, is the reference at these sizes) and the mass drift."""
+    import math
+    import torch
+    from clawpack_amd import problems
+    from clawpack_amd.device import DeviceGrid, make_desc
+
+    out = {}
+    for n in sizes:
+        q, dx, dy = problems.gaussian_humps(n, n, seed)
+        dt = 0.8 * min(dx, dy) / (math.sqrt(float(q[0].max())) + 0.5)
+        grids, rec = {}, {}
+        for dtype_id, name in ((0, "fp64"), (1, "fp32")):
+            d = make_desc(0, dtype_id, n, n, dx, dy, 3, 3, order=2, transverse_waves=2,
+                          mthlim=(4, 4, 4), bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0, dt_variable=False)
+            g = DeviceGrid(d, "cuda:0")
+            g.upload(q)
+            g.init_ctrl(0.0, dt, target_steps=steps)
+            m0 = g.mass_sum(0)
+            drift = 0.0
+            for k in range(steps):
+                g.advance(1)
+                g.sync_cur()
+                drift = max(drift, abs(g.mass_sum(0) - m0) / m0)
+            c = g.read_ctrl()
+            assert c.status == 0 and c.numsteps == steps, "status %d steps %d" % (c.status, c.numsteps)
+            grids[name] = g
+            rec[name] = {"mass_rel_drift_max": drift}
+        a = grids["fp64"].planes()
+        b = grids["fp32"].planes().to(torch.float64)
+        err = {}
+        for m, comp in enumerate(("h", "hu", "hv")):
+            diff = (b[m] - a[m]).abs()
+            err[comp] = {"L1_rel": float(diff.sum() / a[m].abs().sum()),
+                         "Linf_rel": float(diff.max() / a[m].abs().max())}
+        rec["fp32_vs_fp64"] = err
+        rec["steps"] = steps
+        rec["dt"] = dt
+        out["%dx%d" % (n, n)] = rec
+        del grids, a, b
+        torch.cuda.empty_cache()
+    return out
+
+
 def run_adjoint(n, nsteps, seed, fd_dirs):
     """Inversion workload: SWE n x n FP64 periodic, nsteps frozen-dt steps from 1 + eta0, h sampled
     at 64 gauges every 10 steps, checkpoints every 20.  Times the forward (steps + checkpoint
@@ -213,17 +306,32 @@ def run_adjoint(n, nsteps, seed, fd_dirs):
                    "max_rel_err_vs_directional": max(rel_dir) if rel_dir else None}}
 
 
-def cpu_baseline(n, steps, seed):
+def cpu_baseline(n, steps, seed, threads=1):
+    """The C oracle (restatement of step2/flux2/rpn2/rpt2) on the host: same workload at n x n,
+    `steps` accepted steps, `threads` OpenMP threads (slices of a sweep in three colour classes)."""
     from clawpack_amd import problems
     from oracle import corc
     q, dx, dy = problems.gaussian_humps(n, n, seed)
     d = corc.make_desc("swe", n, n, dx, dy, grav=1.0)
     c = corc.make_ctrl(bc_lower=(2, 2), bc_upper=(2, 2))
-    corc.evolve(d, c, q, nsteps=1, dt=1e-4)  # warm the library and page tables
-    t0 = time.perf_counter()
-    _, st = corc.evolve(d, c, q, nsteps=steps, dt=1e-4)
-    el = time.perf_counter() - t0
+    corc.set_threads(threads)
+    try:
+        corc.evolve(d, c, q, nsteps=1, dt=1e-4)  # warm the library and page tables
+        t0 = time.perf_counter()
+        _, st = corc.evolve(d, c, q, nsteps=steps, dt=1e-4)
+        el = time.perf_counter() - t0
+    finally:
+        corc.set_threads(1)
     return n * n * st["numsteps"] / el, el
+
+
+def host_threads():
+    """CPU share for the multi-core baseline: OMP_NUM_THREADS if set, else the core count, at most 16."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(16, n if n > 0 else (os.cpu_count() or 1)))
 
 
 def main():
@@ -276,15 +384,29 @@ def main():
         out["batch"] = {"grids": a.batch_grids, "fp64": run_batch(a.batch_grids, 0, a.batch_steps, a.seq_steps, a.seed),
                         "fp32": run_batch(a.batch_grids, 1, a.batch_steps, a.seq_steps, a.seed)}
         progress("batch done")
+    if not a.no_euler:
+        out["euler"] = run_euler(a.euler_n, a.euler_steps, 10, 1)
+        progress("euler done")
+    if not a.no_precision:
+        out["precision"] = run_precision([int(x) for x in a.precision_sizes.split(",")],
+                                         a.precision_steps, a.seed)
+        progress("precision done")
     if not a.no_adjoint:
         out["adjoint"] = run_adjoint(a.adjoint_n, a.adjoint_steps, a.seed, a.adjoint_fd)
         progress("adjoint done")
     if not a.no_cpu:
-        rate, el = cpu_baseline(a.cpu_n, a.cpu_steps, a.seed)
-        out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-                               "sample": "C oracle (restatement of step2/flux2/rpn2/rpt2), same "
-                                         "workload at %dx%d for %d steps (%.1f s)"
-                                         % (a.cpu_n, a.cpu_n, a.cpu_steps, el)}
+        rate1, el1 = cpu_baseline(a.cpu_n, a.cpu_steps, a.seed, 1)
+        progress("cpu 1 thread done")
+        nth = host_threads()
+        rate_mt, el_mt = cpu_baseline(a.cpu_mt_n, a.cpu_mt_steps, a.seed, nth)
+        progress("cpu %d threads done" % nth)
+        out["cpu_baseline"] = {
+            "value": rate_mt, "unit": "cell-updates/s", "cores": nth, "kind": "port",
+            "sample": "C oracle (CPU restatement of step2/flux2/rpn2/rpt2, not the reference binary), "
+                      "same workload at %dx%d for %d steps on %d OpenMP threads (%.1f s wall)"
+                      % (a.cpu_mt_n, a.cpu_mt_n, a.cpu_mt_steps, nth, el_mt),
+            "single_core": {"value": rate1, "cores": 1,
+                            "sample": "%dx%d for %d steps (%.1f s)" % (a.cpu_n, a.cpu_n, a.cpu_steps, el1)}}
     print(json.dumps(out), flush=True)
 
 

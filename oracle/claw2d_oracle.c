@@ -490,49 +490,90 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
  * qold must hold filled ghosts; qnew receives qold + update (ghost values are scratch).
  * Returns cfl = max over both sweeps.
  * ---------------------------------------------------------------------------------------- */
+/* One x-sweep slice (row j) of step2, accumulated into qnew. */
+static void xsweep_row(const orc_desc *d, int j, const double *qold, double *qnew, orc_work *wk,
+                       double dtdx, double dtdy, double *cfl) {
+    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx;
+    orc_work w = *wk;
+    double cfl1d;
+    for (int i = 1 - g; i <= mx + g; ++i)
+        for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (i + g - 1)] = QB(qold, m, i, j);
+    flux2(d, 1, mx, dtdx, &w, &cfl1d);
+    *cfl = fmax(*cfl, cfl1d);
+    const double *qadd = w.qadd, *fadd = w.fadd, *gadd = w.gadd;
+    for (int i = 1; i <= mx; ++i)
+        for (int m = 0; m < meqn; ++m) {
+            QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, i)
+                                - dtdx * (A1(fadd, m, i + 1) - A1(fadd, m, i))
+                                - dtdy * (GADD(m, 1, i) - GADD(m, 0, i));
+            QB(qnew, m, i, j - 1) = QB(qnew, m, i, j - 1) - dtdy * GADD(m, 0, i);
+            QB(qnew, m, i, j + 1) = QB(qnew, m, i, j + 1) + dtdy * GADD(m, 1, i);
+        }
+}
+
+/* One y-sweep slice (column i) of step2, accumulated into qnew. */
+static void ysweep_col(const orc_desc *d, int i, const double *qold, double *qnew, orc_work *wk,
+                       double dtdx, double dtdy, double *cfl) {
+    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
+    orc_work w = *wk;
+    double cfl1d;
+    for (int j = 1 - g; j <= my + g; ++j)
+        for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (j + g - 1)] = QB(qold, m, i, j);
+    flux2(d, 2, my, dtdy, &w, &cfl1d);
+    *cfl = fmax(*cfl, cfl1d);
+    const double *qadd = w.qadd, *fadd = w.fadd, *gadd = w.gadd;
+    for (int j = 1; j <= my; ++j)
+        for (int m = 0; m < meqn; ++m) {
+            QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, j)
+                                - dtdy * (A1(fadd, m, j + 1) - A1(fadd, m, j))
+                                - dtdx * (GADD(m, 1, j) - GADD(m, 0, j));
+            QB(qnew, m, i - 1, j) = QB(qnew, m, i - 1, j) - dtdx * GADD(m, 0, j);
+            QB(qnew, m, i + 1, j) = QB(qnew, m, i + 1, j) + dtdx * GADD(m, 1, j);
+        }
+}
+
+/* Threads for orc_step2 (1 = the sequential slice order of step2.f90, the default).  With more
+ * threads the slices of one sweep run in three colour classes (slice index mod 3), so no two
+ * concurrent slices update the same cells; sums then associate in a different order than the
+ * sequential loop (round-off level).  Used only to time a multi-core CPU baseline. */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
 int orc_step2(const orc_desc *d, const double *qold, double *qnew, double dt, double *cfl_out) {
     const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
     const int maxm = mx > my ? mx : my;
-    orc_work w;
-    if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves) != 0) { work_free(&w); return -1; }
     const size_t total = (size_t)meqn * (mx + 2 * g) * (my + 2 * g);
     memcpy(qnew, qold, total * sizeof(double));
     const double dtdx = dt / d->dx, dtdy = dt / d->dy;
-    double cfl = 0.0, cfl1d;
-
-    for (int j = 0; j <= my + 1; ++j) {
-        for (int i = 1 - g; i <= mx + g; ++i)
-            for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (i + g - 1)] = QB(qold, m, i, j);
-        flux2(d, 1, mx, dtdx, &w, &cfl1d);
-        cfl = fmax(cfl, cfl1d);
-        const double *qadd = w.qadd, *fadd = w.fadd, *gadd = w.gadd;
-        for (int i = 1; i <= mx; ++i)
-            for (int m = 0; m < meqn; ++m) {
-                QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, i)
-                                    - dtdx * (A1(fadd, m, i + 1) - A1(fadd, m, i))
-                                    - dtdy * (GADD(m, 1, i) - GADD(m, 0, i));
-                QB(qnew, m, i, j - 1) = QB(qnew, m, i, j - 1) - dtdy * GADD(m, 0, i);
-                QB(qnew, m, i, j + 1) = QB(qnew, m, i, j + 1) + dtdy * GADD(m, 1, i);
-            }
-    }
-    for (int i = 0; i <= mx + 1; ++i) {
-        for (int j = 1 - g; j <= my + g; ++j)
-            for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (j + g - 1)] = QB(qold, m, i, j);
-        flux2(d, 2, my, dtdy, &w, &cfl1d);
-        cfl = fmax(cfl, cfl1d);
-        const double *qadd = w.qadd, *fadd = w.fadd, *gadd = w.gadd;
-        for (int j = 1; j <= my; ++j)
-            for (int m = 0; m < meqn; ++m) {
-                QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, j)
-                                    - dtdy * (A1(fadd, m, j + 1) - A1(fadd, m, j))
-                                    - dtdx * (GADD(m, 1, j) - GADD(m, 0, j));
-                QB(qnew, m, i - 1, j) = QB(qnew, m, i - 1, j) - dtdx * GADD(m, 0, j);
-                QB(qnew, m, i + 1, j) = QB(qnew, m, i + 1, j) + dtdx * GADD(m, 1, j);
+    double cfl = 0.0;
+    int err = 0;
+    if (g_threads <= 1) {
+        orc_work w;
+        if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves) != 0) { work_free(&w); return -1; }
+        for (int j = 0; j <= my + 1; ++j) xsweep_row(d, j, qold, qnew, &w, dtdx, dtdy, &cfl);
+        for (int i = 0; i <= mx + 1; ++i) ysweep_col(d, i, qold, qnew, &w, dtdx, dtdy, &cfl);
+        work_free(&w);
+    } else {
+        for (int dir = 0; dir < 2; ++dir)
+            for (int colour = 0; colour < 3; ++colour) {
+                const int last = dir == 0 ? my + 1 : mx + 1;
+#pragma omp parallel num_threads(g_threads) reduction(max : cfl) reduction(| : err)
+                {
+                    orc_work w;
+                    if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves) != 0) err = 1;
+                    else {
+#pragma omp for schedule(static)
+                        for (int k = colour; k <= last; k += 3) {
+                            if (dir == 0) xsweep_row(d, k, qold, qnew, &w, dtdx, dtdy, &cfl);
+                            else ysweep_col(d, k, qold, qnew, &w, dtdx, dtdy, &cfl);
+                        }
+                    }
+                    work_free(&w);
+                }
             }
     }
     *cfl_out = cfl;
-    work_free(&w);
-    return 0;
+    return err ? -1 : 0;
 }
 
 /* ------------------------------------------------------------------------------------------

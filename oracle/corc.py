@@ -69,6 +69,8 @@ def lib():
                                 dpa, dpa]
         L.orc_philim.argtypes = [C.c_double, C.c_double, C.c_int]
         L.orc_philim.restype = C.c_double
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = None
         L.orc_component_sum.argtypes = [C.POINTER(Desc), dpa, C.c_int]
         L.orc_component_sum.restype = C.c_double
         for f in ("orc_step2", "orc_step_hyperbolic", "orc_evolve", "orc_rpn2", "orc_rpt2",
@@ -76,6 +78,12 @@ def lib():
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Threads for the step (1 = sequential slice order, the default; n > 1 only for timing a
+    multi-core CPU baseline: sums then associate differently at round-off level)."""
+    lib().orc_set_threads(int(n))
 
 
 def _p(a):

@@ -325,3 +325,34 @@ def test_c_oracle_equals_torch_restatement_transonic(amp, bc):
         qt, _ = ts.step(torch.tensor(q), dt, dx, dy, "swe", G, mthlim=(4, 4, 4), bc_lower=bc,
                         bc_upper=bc, order=order, transverse_waves=tw)
         assert np.abs(qc - qt.numpy()).max() <= 1e-13 * np.abs(q).max()
+
+
+def test_threaded_oracle_matches_sequential():
+    """The multi-core baseline path (slices in three colour classes on 4 threads) computes the
+    same step as the sequential slice order, up to the association of sums."""
+    q = swe_state(47, 38, seed=3)
+    d = corc.make_desc("swe", 47, 38, 0.1, 0.11, grav=G)
+    c = corc.make_ctrl(bc_lower=(2, 3), bc_upper=(2, 3))
+    q1, st1 = corc.evolve(d, c, q, nsteps=5)
+    corc.set_threads(4)
+    try:
+        q4, st4 = corc.evolve(d, c, q, nsteps=5)
+    finally:
+        corc.set_threads(1)
+    assert st1["numsteps"] == st4["numsteps"] == 5
+    assert np.abs(q1 - q4).max() <= 1e-13 * np.abs(q).max()
+
+
+def test_euler_quadrants_roundoff_sensitivity():
+    """Conditioning of the 500-step Euler quadrant run: perturbing q0 by one ulp (relative 2.2e-16,
+    random signs) moves the oracle's own result by more than 1e-12 max|q| at 96^2 (1.7e-11 at
+    160^2).  Differences of that size between two correct FP64 implementations that round
+    differently are therefore expected; the GPU parity test of this run uses 1e-11."""
+    n = 96
+    q, dx, dy = problems.euler_quadrants(n, n, 1)
+    d = corc.make_desc("euler", n, n, dx, dy, gamma=GAMMA)
+    qa, sa = corc.evolve(d, corc.make_ctrl(), q, nsteps=500)
+    rng = np.random.default_rng(0)
+    qb, sb = corc.evolve(d, corc.make_ctrl(), q * (1 + 2.2e-16 * rng.choice([-1, 1], size=q.shape)), nsteps=500)
+    assert sa["numsteps"] == sb["numsteps"] == 500
+    assert np.abs(qa - qb).max() > 1e-12 * np.abs(q).max()
