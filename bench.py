@@ -228,10 +228,15 @@ def run_precision(sizes, steps, seed):
         a = grids["fp64"].planes()
         b = grids["fp32"].planes().to(torch.float64)
         err = {}
+        qmax = float(a.abs().max())
+        dall = (b - a).abs()
         for m, comp in enumerate(("h", "hu", "hv")):
-            diff = (b[m] - a[m]).abs()
+            diff = dall[m]
             err[comp] = {"L1_rel": float(diff.sum() / a[m].abs().sum()),
                          "Linf_rel": float(diff.max() / a[m].abs().max())}
+        # the state as a whole: max |dq| / max |q| and sum |dq| / sum |q| over all components
+        err["q"] = {"Linf_rel_to_max_q": float(dall.max()) / qmax,
+                    "L1_rel": float(dall.sum() / a.abs().sum())}
         rec["fp32_vs_fp64"] = err
         rec["steps"] = steps
         rec["dt"] = dt

@@ -164,23 +164,32 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
 #pragma unroll
   for (int m = 0; m < NEQ; ++m) LG1[m] = LG2[m] = RG1[m] = RG2[m] = T(0);
   if (a.tw > 0) {
-    T sg[NW], bA[NW], bB[NW], c1[NW], c2[NW], c3[NW], c4[NW];
-    P::tspeeds(f, sg);
+    T bA[NW], bB[NW];
     P::tbeta(f, A, bA);
     P::tbeta(f, B, bB);
     const T w = T(-0.5) * r * rt;
+    // transversely subsonic on every lane (v - a < 0 < v + a): the down/up split has a fixed
+    // pattern, so the zero entries are skipped; the result is the same bit for bit
+    const bool sub = f.v - f.a < T(0) && f.v + f.a > T(0);
+    if (__ballot(!sub) == 0ull) {
+      P::expand_t_split(f, w, bA, LG1, LG2);
+      P::expand_t_split(f, w, bB, RG1, RG2);
+    } else {
+      T sg[NW], c1[NW], c2[NW], c3[NW], c4[NW];
+      P::tspeeds(f, sg);
 #pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      const T wm = w * vmin(sg[j], T(0)), wp = w * vmax(sg[j], T(0));
-      c1[j] = wm * bA[j];
-      c2[j] = wp * bA[j];
-      c3[j] = wm * bB[j];
-      c4[j] = wp * bB[j];
+      for (int j = 0; j < NW; ++j) {
+        const T wm = w * vmin(sg[j], T(0)), wp = w * vmax(sg[j], T(0));
+        c1[j] = wm * bA[j];
+        c2[j] = wp * bA[j];
+        c3[j] = wm * bB[j];
+        c4[j] = wp * bB[j];
+      }
+      P::expand_t(f, c1, LG1);
+      P::expand_t(f, c2, LG2);
+      P::expand_t(f, c3, RG1);
+      P::expand_t(f, c4, RG2);
     }
-    P::expand_t(f, c1, LG1);
-    P::expand_t(f, c2, LG2);
-    P::expand_t(f, c3, RG1);
-    P::expand_t(f, c4, RG2);
   }
 }
 
@@ -264,45 +273,21 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
     if (R <= y1) load_row(R + 1, qnext, fnext);
     const Cell cR = P::prep(qR, a.par);
 
+    // ---- request the left neighbour's cell data for the x-interfaces of row R now, so that the
+    // y work below hides the ds_bpermute latency
+    T pl[NC];
+    {
+      T pk[NC];
+      P::pack(cR, pk);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) pl[k] = shfl(pk[k], addrL);
+    }
+
     // ---- y-interface R (between rows R-1 and R), in-lane
     Iface fynew;
     if (R >= y0 - 1) {
       P::template rpn2<1>(cprev, cR, a.par, fynew);
       if (ycfl_col && R >= 0 && R <= my) smy = nmax(smy, P::smax(fynew));
-    }
-
-    // ---- x-interfaces of row R
-    if (R >= y0 - 1 && R <= y1) {
-      T pk[NC], pl[NC];
-      P::pack(cR, pk);
-#pragma unroll
-      for (int k = 0; k < NC; ++k) pl[k] = shfl(pk[k], addrL);
-      const Cell cL = P::unpack(pl);
-      Iface fxi;
-      P::template rpn2<0>(cL, cR, a.par, fxi);
-      if (xcfl_col && R >= -1 && R <= my) smx = nmax(smx, P::smax(fxi));
-      T lself[NL], lnb[NL], mR[NW], mL[NW], alR[NW], alL[NW];
-      P::lim(fxi, lself);
-#pragma unroll
-      for (int k = 0; k < NL; ++k) lnb[k] = shfl(lself[k], addrR);
-      P::mdot(lself, lnb, mR);
-#pragma unroll
-      for (int p = 0; p < NW; ++p) {
-        alR[p] = lnb[p];
-        alL[p] = shfl(fxi.al[p], addrL);
-        mL[p] = P::unit_m(p) ? T(1) : shfl(mR[p], addrL);
-      }
-      T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-      finish_iface<T, P, LIM>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2);
-#pragma unroll
-      for (int m = 0; m < NEQ; ++m) {
-        const T X = RX[m] + shfl(LX[m], addrR);
-        const T G1 = RG1[m] + shfl(LG1[m], addrR);
-        const T G2 = RG2[m] + shfl(LG2[m], addrR);
-        acc1[m] -= G1;
-        acc2[m] += (X + G1) - G2;
-        acc3[m] += G2;
-      }
     }
 
     // ---- finish y-interface R-1 (between rows R-2 and R-1)
@@ -346,6 +331,42 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       for (int p = 0; p < NW; ++p) {
         ymprev[p] = mR[p];
         yal2[p] = fyprev.al[p];
+      }
+    }
+
+    // ---- x-interfaces of row R (the left cells were requested before the y work)
+    if (R >= y0 - 1 && R <= y1) {
+      const Cell cL = P::unpack(pl);
+      Iface fxi;
+      P::template rpn2<0>(cL, cR, a.par, fxi);
+      if (xcfl_col && R >= -1 && R <= my) smx = nmax(smx, P::smax(fxi));
+      // limiter partners: both neighbours' wave data in one exchange round; the left pair's
+      // eigenvector products are recomputed here (the same operands in the same order as on the
+      // left lane, so the same bits)
+      T lself[NL], lnb[NL], lleft[NL], mR[NW], mL[NW], alR[NW], alL[NW];
+      P::lim(fxi, lself);
+#pragma unroll
+      for (int k = 0; k < NL; ++k) {
+        lnb[k] = shfl(lself[k], addrR);
+        lleft[k] = shfl(lself[k], addrL);
+      }
+      P::mdot(lself, lnb, mR);
+      P::mdot(lleft, lself, mL);
+#pragma unroll
+      for (int p = 0; p < NW; ++p) {
+        alR[p] = lnb[p];
+        alL[p] = lleft[p];
+      }
+      T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
+      finish_iface<T, P, LIM>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2);
+#pragma unroll
+      for (int m = 0; m < NEQ; ++m) {
+        const T X = RX[m] + shfl(LX[m], addrR);
+        const T G1 = RG1[m] + shfl(LG1[m], addrR);
+        const T G2 = RG2[m] + shfl(LG2[m], addrR);
+        acc1[m] -= G1;
+        acc2[m] += (X + G1) - G2;
+        acc3[m] += G2;
       }
     }
 
@@ -459,8 +480,20 @@ __device__ __forceinline__ void control_one(claw2d_ctrl* c, double dx, double dy
   c->dt = ndt;
 }
 
+// Minimum resident wavefronts per SIMD the register allocator must allow (0 = its own choice).
+#ifndef CLAW_MIN_WAVES_F64
+#define CLAW_MIN_WAVES_F64 0
+#endif
+#ifndef CLAW_MIN_WAVES_F32
+#define CLAW_MIN_WAVES_F32 0
+#endif
+template <class T> struct MinWaves {
+  static constexpr int value = sizeof(T) == 8 ? CLAW_MIN_WAVES_F64 : CLAW_MIN_WAVES_F32;
+};
+
 template <class T, class P, int LIM>
-__global__ void __launch_bounds__(kLanes* kWPB) step2_fused(StepArgs<T, P> a) {
+__global__ void __launch_bounds__(kLanes* kWPB) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::value)))
+step2_fused(StepArgs<T, P> a) {
   double dt = a.dt_fixed;
   const T* qin = a.q0;
   T* qout = a.q1;

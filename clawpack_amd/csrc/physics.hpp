@@ -210,7 +210,22 @@ template <class T> struct SweP {
   __device__ __forceinline__ static void expand_t(const Iface& f, const T c[NW], T o[NEQ]) {
     o[0] = c[0] + c[2];
     o[1] = fma(o[0], f.u, c[1]);
-    o[2] = c[0] * (f.v - f.a) + c[2] * (f.v + f.a);
+    o[2] = fma(c[2], f.v + f.a, c[0] * (f.v - f.a));
+  }
+  // expand_t of the down-going (wm) and up-going (wp) halves of w*beta when v - a < 0 < v + a:
+  // wave 1 is down-going, wave 3 up-going, the shear wave goes with the sign of v.  Bitwise equal
+  // to expand_t of the general split, whose other entries are (signed) zeros.
+  __device__ __forceinline__ static void expand_t_split(const Iface& f, T w, const T b[NW], T Gm[NEQ],
+                                                        T Gp[NEQ]) {
+    const T c0 = (w * (f.v - f.a)) * b[0];
+    const T c1m = (w * vmin(f.v, T(0))) * b[1], c1p = (w * vmax(f.v, T(0))) * b[1];
+    const T c2 = (w * (f.v + f.a)) * b[2];
+    Gm[0] = c0;
+    Gm[1] = fma(c0, f.u, c1m);
+    Gm[2] = c0 * (f.v - f.a);
+    Gp[0] = c2;
+    Gp[1] = fma(c2, f.u, c1p);
+    Gp[2] = c2 * (f.v + f.a);
   }
 };
 
@@ -368,12 +383,30 @@ template <class T> struct EulerP {
     sg[0] = f.v - f.a; sg[1] = f.v; sg[2] = f.v; sg[3] = f.v + f.a;
   }
   __device__ __forceinline__ static void expand_t(const Iface& f, const T c[NW], T o[NEQ]) {
-    const T sum = c[0] + c[1] + c[3];
+    const T k = T(0.5) * (f.u * f.u + f.v * f.v);
+    const T sum = (c[0] + c[1]) + c[3];
     o[0] = sum;
     o[1] = fma(sum, f.u, c[2]);
-    o[2] = c[0] * (f.v - f.a) + c[1] * f.v + c[3] * (f.v + f.a);
-    o[3] = c[0] * (f.H - f.v * f.a) + c[1] * (T(0.5) * (f.u * f.u + f.v * f.v)) + c[2] * f.u +
-           c[3] * (f.H + f.v * f.a);
+    o[2] = fma(c[3], f.v + f.a, fma(c[1], f.v, c[0] * (f.v - f.a)));
+    o[3] = fma(c[3], f.H + f.v * f.a, fma(c[2], f.u, fma(c[1], k, c[0] * (f.H - f.v * f.a))));
+  }
+  // expand_t of the down-going and up-going halves of w*beta when v - a < 0 < v + a (wave 1
+  // down, wave 4 up, entropy and shear waves with the sign of v); bitwise equal to expand_t of the
+  // general split, whose other entries are (signed) zeros.
+  __device__ __forceinline__ static void expand_t_split(const Iface& f, T w, const T b[NW], T Gm[NEQ],
+                                                        T Gp[NEQ]) {
+    const T k = T(0.5) * (f.u * f.u + f.v * f.v);
+    const T wvm = w * vmin(f.v, T(0)), wvp = w * vmax(f.v, T(0));
+    const T c0 = (w * (f.v - f.a)) * b[0], c3 = (w * (f.v + f.a)) * b[3];
+    const T c1m = wvm * b[1], c2m = wvm * b[2], c1p = wvp * b[1], c2p = wvp * b[2];
+    Gm[0] = c0 + c1m;
+    Gm[1] = fma(Gm[0], f.u, c2m);
+    Gm[2] = fma(c1m, f.v, c0 * (f.v - f.a));
+    Gm[3] = fma(c2m, f.u, fma(c1m, k, c0 * (f.H - f.v * f.a)));
+    Gp[0] = c1p + c3;
+    Gp[1] = fma(Gp[0], f.u, c2p);
+    Gp[2] = fma(c3, f.v + f.a, c1p * f.v);
+    Gp[3] = fma(c3, f.H + f.v * f.a, fma(c2p, f.u, c1p * k));
   }
 };
 
