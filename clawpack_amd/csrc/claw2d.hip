@@ -210,6 +210,9 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
                                            const int x0, const int y0, const int y1, const double dt,
                                            const double dx, const double dy, claw2d_ctrl* ctrl) {
   constexpr int NEQ = P::NEQ, NW = P::NW, NC = P::NC, NL = P::NL;
+  // 4 equations: add each y-interface's upper-cell shares at once (two more exchanges per row)
+  // instead of carrying them to the next row, so the kernel fits 2 wavefronts per SIMD
+  constexpr bool kImmY = NEQ == 4;
   using Cell = typename P::Cell;
   using Iface = typename P::Iface;
   const int lane = threadIdx.x & 63;
@@ -248,15 +251,15 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
 
   T smx = T(0), smy = T(0);
   bool bad = false;
-  Cell cprev;
+  Cell cprev = {};
   Iface fyprev;
   T yal2[NW], ymprev[NW];  // y: strengths of interface R-2, e-products of the pair (R-2, R-1)
-  T qm1[NEQ], qm2[NEQ], qnext[NEQ];
+  T qm2[NEQ], qnext[NEQ];  // q of row R-2 (the row being finished), prefetched row R+1
   T acc0[NEQ], acc1[NEQ], acc2[NEQ], acc3[NEQ];
   T yX[NEQ], yG1[NEQ], yG2[NEQ];
 #pragma unroll
   for (int m = 0; m < NEQ; ++m) {
-    qm1[m] = qm2[m] = T(0);
+    qm2[m] = T(0);
     acc0[m] = acc1[m] = acc2[m] = acc3[m] = T(0);
     yX[m] = yG1[m] = yG2[m] = T(0);
   }
@@ -302,16 +305,28 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
                                 RX, RG1, RG2);
         yswap<NEQ>(LX); yswap<NEQ>(LG1); yswap<NEQ>(LG2);
         yswap<NEQ>(RX); yswap<NEQ>(RG1); yswap<NEQ>(RG2);
-        if (R >= y0 + 2) {
-          // cell (c, R-2) has all its y-contributions
+        if constexpr (kImmY) {
+          // both cells' shares go into their accumulators now (rows R-2 and R-1)
 #pragma unroll
           for (int m = 0; m < NEQ; ++m) {
-            const T Y = yX[m] + LX[m];
-            const T G1 = yG1[m] + LG1[m];
-            const T G2 = yG2[m] + LG2[m];
-            const T g2l = shfl(G2, addrL);
-            const T g1r = shfl(G1, addrR);
-            acc0[m] += ((Y + G1) - G2) + (g2l - g1r);
+            const T l2 = shfl(LG2[m], addrL), l1 = shfl(LG1[m], addrR);
+            const T r2 = shfl(RG2[m], addrL), r1 = shfl(RG1[m], addrR);
+            acc0[m] += ((LX[m] + LG1[m]) - LG2[m]) + (l2 - l1);
+            acc1[m] += ((RX[m] + RG1[m]) - RG2[m]) + (r2 - r1);
+          }
+        }
+        if (R >= y0 + 2) {
+          // cell (c, R-2) has all its y-contributions
+          if constexpr (!kImmY) {
+#pragma unroll
+            for (int m = 0; m < NEQ; ++m) {
+              const T Y = yX[m] + LX[m];
+              const T G1 = yG1[m] + LG1[m];
+              const T G2 = yG2[m] + LG2[m];
+              const T g2l = shfl(G2, addrL);
+              const T g1r = shfl(G1, addrR);
+              acc0[m] += ((Y + G1) - G2) + (g2l - g1r);
+            }
           }
           if (own_col) {
             T qn[NEQ];
@@ -324,8 +339,10 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
             bad |= P::bad(qn);
           }
         }
+        if constexpr (!kImmY) {
 #pragma unroll
-        for (int m = 0; m < NEQ; ++m) { yX[m] = RX[m]; yG1[m] = RG1[m]; yG2[m] = RG2[m]; }
+          for (int m = 0; m < NEQ; ++m) { yX[m] = RX[m]; yG1[m] = RG1[m]; yG2[m] = RG2[m]; }
+        }
       }
 #pragma unroll
       for (int p = 0; p < NW; ++p) {
@@ -373,8 +390,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
     // ---- roll the column window
 #pragma unroll
     for (int m = 0; m < NEQ; ++m) {
-      qm2[m] = qm1[m];
-      qm1[m] = qR[m];
+      qm2[m] = cprev.q[m];  // row R-1 (cprev still holds it)
       acc0[m] = acc1[m];
       acc1[m] = acc2[m];
       acc2[m] = acc3[m];
@@ -482,7 +498,7 @@ __device__ __forceinline__ void control_one(claw2d_ctrl* c, double dx, double dy
 
 // Minimum resident wavefronts per SIMD the register allocator must allow (0 = its own choice).
 #ifndef CLAW_MIN_WAVES_F64
-#define CLAW_MIN_WAVES_F64 0
+#define CLAW_MIN_WAVES_F64 2
 #endif
 #ifndef CLAW_MIN_WAVES_F32
 #define CLAW_MIN_WAVES_F32 0
@@ -542,7 +558,8 @@ struct BatchDev {
 };
 
 template <class T, class P, int LIM>
-__global__ void __launch_bounds__(kLanes* kWPB) batch_step(StepArgs<T, P> a, BatchDev b) {
+__global__ void __launch_bounds__(kLanes* kWPB) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::value)))
+batch_step(StepArgs<T, P> a, BatchDev b) {
   const int lane = threadIdx.x & 63;
   for (;;) {
     // every lane takes part (lane 0 adds 1, the others 0) and lane 0's ticket is broadcast, so

@@ -235,11 +235,12 @@ template <class T> struct SweP {
 // f1 = (1, u, v-a, H-va), f2 = (1, u, v, q2/2), f3 = (0, 1, 0, u), f4 = (1, u, v+a, H+va).
 // ------------------------------------------------------------------------------------------
 template <class T> struct EulerP {
-  static constexpr int NEQ = 4, NW = 4, NC = 10, NL = 8;
+  static constexpr int NEQ = 4, NW = 4, NC = 9, NL = 8;
   __device__ __forceinline__ static constexpr bool unit_m(int) { return false; }
   struct Par { T gamma, gamma1; };
-  // per-cell: q, sqrt(rho), 1/sqrt(rho), rho u/sqrt(rho), rho v/sqrt(rho), (E+p)/sqrt(rho), c
-  struct Cell { T q[4]; T s, rs, P, Q, Hs, c; };
+  // per-cell: q, 1/sqrt(rho), rho u/sqrt(rho), rho v/sqrt(rho), (E+p)/sqrt(rho), c
+  // (sqrt(rho) = rho/sqrt(rho) is formed where it is used: one register pair less per cell)
+  struct Cell { T q[4]; T rs, P, Q, Hs, c; };
   struct Iface { T al[4]; T sf[4]; T u, v, a, H, i2a; };
 
   __device__ __forceinline__ static bool bad(const T q[4]) { return !(q[0] > T(0)); }
@@ -248,7 +249,6 @@ template <class T> struct EulerP {
     Cell c;
     c.q[0] = q[0]; c.q[1] = q[1]; c.q[2] = q[2]; c.q[3] = q[3];
     c.rs = frsq(q[0]);
-    c.s = q[0] * c.rs;
     c.P = q[1] * c.rs;
     c.Q = q[2] * c.rs;
     const T p = pr.gamma1 * (q[3] - T(0.5) * (c.P * c.P + c.Q * c.Q));
@@ -259,12 +259,12 @@ template <class T> struct EulerP {
   }
   __device__ __forceinline__ static void pack(const Cell& c, T o[NC]) {
     o[0] = c.q[0]; o[1] = c.q[1]; o[2] = c.q[2]; o[3] = c.q[3];
-    o[4] = c.s; o[5] = c.rs; o[6] = c.P; o[7] = c.Q; o[8] = c.Hs; o[9] = c.c;
+    o[4] = c.rs; o[5] = c.P; o[6] = c.Q; o[7] = c.Hs; o[8] = c.c;
   }
   __device__ __forceinline__ static Cell unpack(const T o[NC]) {
     Cell c;
     c.q[0] = o[0]; c.q[1] = o[1]; c.q[2] = o[2]; c.q[3] = o[3];
-    c.s = o[4]; c.rs = o[5]; c.P = o[6]; c.Q = o[7]; c.Hs = o[8]; c.c = o[9];
+    c.rs = o[4]; c.P = o[5]; c.Q = o[6]; c.Hs = o[7]; c.c = o[8];
     return c;
   }
 
@@ -285,7 +285,7 @@ template <class T> struct EulerP {
     constexpr int mu = DIR == 0 ? 1 : 2, mv = DIR == 0 ? 2 : 1;
     const T pnL = DIR == 0 ? L.P : L.Q, ptL = DIR == 0 ? L.Q : L.P;
     const T pnR = DIR == 0 ? R.P : R.Q, ptR = DIR == 0 ? R.Q : R.P;
-    const T inv = frcp(L.s + R.s);
+    const T inv = frcp(L.q[0] * L.rs + R.q[0] * R.rs);
     const T u = (pnL + pnR) * inv;
     const T v = (ptL + ptR) * inv;
     const T H = (L.Hs + R.Hs) * inv;
