@@ -55,6 +55,7 @@ typedef struct {
     int max_steps;
     int bc_lower[2], bc_upper[2];
     int reflect_index[2]; /* 0-based q component negated at a wall, per dimension */
+    int dimensional_split; /* 1: Godunov splitting (step2ds), 0: unsplit (step2) */
 } orc_ctrl;
 
 typedef struct {
@@ -650,6 +651,66 @@ void orc_fill_ghosts(const orc_desc *d, const orc_ctrl *c, const double *q, doub
 /* One hyperbolic step on interior q (num_eqn, mx, my Fortran order), in place.
  * Follows ClawSolver2D.step_hyperbolic (pyclaw/src/pyclaw/classic/solver.py, upstream,
  * absent): fill ghosts, qold = qbc copy, step2, copy the interior back. */
+/* ------------------------------------------------------------------------------------------
+ * step2ds: one sweep direction of the dimensionally split step.  Follows
+ * pyclaw/src/pyclaw/classic/step2ds.f90 (upstream, absent): flux2 on every interior slice of
+ * direction ids (1: rows, x-sweeps; 2: columns, y-sweeps) with transverse propagation off
+ * (method(3) < 0), qnew += qadd - dt/dx (fadd(i+1) - fadd(i)).  Returns the sweep's CFL.
+ * ---------------------------------------------------------------------------------------- */
+int orc_step2ds(const orc_desc *d, const double *qold, double *qnew, double dt, int ids,
+                double *cfl_out) {
+    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
+    const int maxm = mx > my ? mx : my;
+    orc_desc dd = *d;
+    dd.transverse_waves = 0;
+    orc_work w;
+    if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves) != 0) { work_free(&w); return -1; }
+    const size_t total = (size_t)meqn * (mx + 2 * g) * (my + 2 * g);
+    memcpy(qnew, qold, total * sizeof(double));
+    double cfl = 0.0, cfl1d;
+    if (ids == 1) {
+        const double dtdx = dt / d->dx;
+        for (int j = 1; j <= my; ++j) {
+            for (int i = 1 - g; i <= mx + g; ++i)
+                for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (i + g - 1)] = QB(qold, m, i, j);
+            flux2(&dd, 1, mx, dtdx, &w, &cfl1d);
+            cfl = fmax(cfl, cfl1d);
+            const double *qadd = w.qadd, *fadd = w.fadd;
+            for (int i = 1; i <= mx; ++i)
+                for (int m = 0; m < meqn; ++m)
+                    QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, i)
+                                        - dtdx * (A1(fadd, m, i + 1) - A1(fadd, m, i));
+        }
+    } else {
+        const double dtdy = dt / d->dy;
+        for (int i = 1; i <= mx; ++i) {
+            for (int j = 1 - g; j <= my + g; ++j)
+                for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (j + g - 1)] = QB(qold, m, i, j);
+            flux2(&dd, 2, my, dtdy, &w, &cfl1d);
+            cfl = fmax(cfl, cfl1d);
+            const double *qadd = w.qadd, *fadd = w.fadd;
+            for (int j = 1; j <= my; ++j)
+                for (int m = 0; m < meqn; ++m)
+                    QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, j)
+                                        - dtdy * (A1(fadd, m, j + 1) - A1(fadd, m, j));
+        }
+    }
+    *cfl_out = cfl;
+    work_free(&w);
+    return 0;
+}
+
+static void copy_interior(const orc_desc *d, const double *qbc, double *q) {
+    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
+    for (int j = 1; j <= my; ++j)
+        for (int i = 1; i <= mx; ++i)
+            for (int m = 0; m < meqn; ++m)
+                q[m + meqn * ((i - 1) + (size_t)mx * (j - 1))] = QB(qbc, m, i, j);
+}
+
+/* ClawSolver2D.step_hyperbolic (classic/solver.py, upstream): ghost fill, then step2, or with
+ * dimensional splitting the x-sweeps of step2ds, a fresh ghost fill of the intermediate state and
+ * the y-sweeps (Godunov splitting, x first); cfl = max of the two sweeps. */
 int orc_step_hyperbolic(const orc_desc *d, const orc_ctrl *c, double *q, double dt, double *cfl) {
     const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
     const size_t nbc = (size_t)meqn * (mx + 2 * g) * (my + 2 * g);
@@ -657,12 +718,20 @@ int orc_step_hyperbolic(const orc_desc *d, const orc_ctrl *c, double *q, double 
     double *qnew = malloc(nbc * sizeof(double));
     if (!qold || !qnew) { free(qold); free(qnew); return -1; }
     orc_fill_ghosts(d, c, q, qold);
-    int rc = orc_step2(d, qold, qnew, dt, cfl);
-    if (rc == 0)
-        for (int j = 1; j <= my; ++j)
-            for (int i = 1; i <= mx; ++i)
-                for (int m = 0; m < meqn; ++m)
-                    q[m + meqn * ((i - 1) + (size_t)mx * (j - 1))] = QB(qnew, m, i, j);
+    int rc;
+    if (c->dimensional_split) {
+        double cx = 0.0, cy = 0.0;
+        rc = orc_step2ds(d, qold, qnew, dt, 1, &cx);
+        if (rc == 0) {
+            copy_interior(d, qnew, q);
+            orc_fill_ghosts(d, c, q, qold);
+            rc = orc_step2ds(d, qold, qnew, dt, 2, &cy);
+        }
+        *cfl = fmax(cx, cy);
+    } else {
+        rc = orc_step2(d, qold, qnew, dt, cfl);
+    }
+    if (rc == 0) copy_interior(d, qnew, q);
     free(qold);
     free(qnew);
     return rc;

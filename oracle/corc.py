@@ -30,6 +30,7 @@ class Ctrl(C.Structure):
         ("dt_initial", C.c_double), ("dt_max", C.c_double), ("cfl_max", C.c_double),
         ("cfl_desired", C.c_double), ("dt_variable", C.c_int), ("max_steps", C.c_int),
         ("bc_lower", C.c_int * 2), ("bc_upper", C.c_int * 2), ("reflect_index", C.c_int * 2),
+        ("dimensional_split", C.c_int),
     ]
 
 
@@ -106,10 +107,12 @@ def make_desc(solver, mx, my, dx, dy, grav=1.0, gamma=1.4, order=2, transverse_w
 
 
 def make_ctrl(bc_lower=(1, 1), bc_upper=(1, 1), reflect_index=(1, 2), dt_initial=0.1,
-              dt_max=1e99, cfl_max=1.0, cfl_desired=0.9, dt_variable=True, max_steps=10000):
+              dt_max=1e99, cfl_max=1.0, cfl_desired=0.9, dt_variable=True, max_steps=10000,
+              dimensional_split=False):
     c = Ctrl()
     c.dt_initial, c.dt_max, c.cfl_max, c.cfl_desired = dt_initial, dt_max, cfl_max, cfl_desired
     c.dt_variable, c.max_steps = int(bool(dt_variable)), int(max_steps)
+    c.dimensional_split = int(bool(dimensional_split))
     for k in range(2):
         c.bc_lower[k], c.bc_upper[k], c.reflect_index[k] = bc_lower[k], bc_upper[k], reflect_index[k]
     return c

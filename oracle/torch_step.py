@@ -323,3 +323,19 @@ def step(q, dt, dx, dy, solver, phys, mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_u
     """ClawSolver2D.step_hyperbolic on interior q (meqn, mx, my) -> (qnew, cfl)."""
     qbc = fill_ghosts(q, g, bc_lower, bc_upper, reflect_index)
     return step2(qbc, dt, dx, dy, g, solver, phys, mthlim, order, transverse_waves)
+
+
+def step_split(q, dt, dx, dy, solver, phys, mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_upper=(1, 1),
+               reflect_index=(1, 2), g=2, order=2):
+    """Dimensionally split step (Godunov: x-sweeps, ghost refill, y-sweeps; no transverse
+    propagation) on interior q (meqn, mx, my) -> (qnew, cfl = max of the two sweeps)."""
+    meqn, mx, my = q.shape
+    dtdx, dtdy = dt / dx, dt / dy
+    qbc = fill_ghosts(q, g, bc_lower, bc_upper, reflect_index)
+    qa, fd, _, _, smx = sweep(qbc[:, :, g:g + my], dtdx, g, solver, phys, mthlim, order, 0)
+    q1 = q + qa - dtdx * fd
+    qbc = fill_ghosts(q1, g, bc_lower, bc_upper, reflect_index)
+    qa, fd, _, _, smy = sweep(_swap_xy(qbc[:, g:g + mx, :], solver), dtdy, g, solver, phys, mthlim,
+                              order, 0)
+    q2 = q1 + _swap_xy(qa - dtdy * fd, solver)
+    return q2, torch.maximum(dtdx * smx, dtdy * smy)

@@ -356,3 +356,85 @@ def test_euler_quadrants_roundoff_sensitivity():
     qb, sb = corc.evolve(d, corc.make_ctrl(), q * (1 + 2.2e-16 * rng.choice([-1, 1], size=q.shape)), nsteps=500)
     assert sa["numsteps"] == sb["numsteps"] == 500
     assert np.abs(qa - qb).max() > 1e-12 * np.abs(q).max()
+
+
+# ---- dimensionally split step (step2ds; pyclaw's default in 2D) ----------------------------
+
+def split_step(solver, q, dt, dx=0.1, dy=0.11, bc=(1, 1), **kw):
+    mx, my = q.shape[1:]
+    d = corc.make_desc(solver, mx, my, dx, dy, grav=G, gamma=GAMMA, **kw)
+    c = corc.make_ctrl(bc_lower=bc, bc_upper=bc, dimensional_split=True)
+    return corc.step_hyperbolic(d, c, q, dt)
+
+
+@pytest.mark.parametrize("solver", ["swe", "euler"])
+@pytest.mark.parametrize("bc", [(1, 1), (2, 2), (3, 3), (1, 3)])
+@pytest.mark.parametrize("order,lim", [(2, (4, 4, 4, 4)), (2, (1, 2, 3, 5)), (1, (0,) * 4)])
+def test_split_c_oracle_equals_torch_restatement(solver, bc, order, lim):
+    mx, my = 15, 12
+    q = swe_state(mx, my, seed=6) if solver == "swe" else euler_state(mx, my, seed=6)
+    dx, dy, dt = 0.1, 0.11, 0.013
+    qc, cc = split_step(solver, q, dt, dx, dy, bc, order=order, mthlim=lim)
+    qt, ct = ts.step_split(torch.tensor(q), dt, dx, dy, solver, G if solver == "swe" else GAMMA,
+                           mthlim=lim, bc_lower=bc, bc_upper=bc, order=order)
+    assert np.abs(qc - qt.numpy()).max() <= 1e-13 * np.abs(q).max()
+    assert abs(cc - float(ct)) <= 1e-15 * cc
+
+
+@pytest.mark.parametrize("solver", ["swe", "euler"])
+def test_split_one_dimensional_data(solver):
+    """Data that vary only in x: the y-sweeps change nothing and the split step equals the unsplit
+    step (whose transverse terms cancel between identical rows); likewise for data varying in y."""
+    mx, my = 24, 6
+    q = swe_state(mx, 1, seed=2) if solver == "swe" else euler_state(mx, 1, seed=2)
+    qx = np.repeat(q, my, axis=2)
+    qx[2] = 0.0  # no transverse momentum
+    qs, cs = split_step(solver, qx, 0.01, bc=(2, 2))
+    qu, cu = one_step(solver, qx, 0.01, bc=(2, 2), dx=0.1, dy=0.11)
+    assert np.abs(qs - qu).max() <= 1e-14 * np.abs(qx).max()
+    # the same data turned by 90 degrees
+    qy = np.ascontiguousarray(qx.transpose(0, 2, 1))
+    qy[[1, 2]] = qy[[2, 1]]
+    qs2, _ = split_step(solver, qy, 0.01, dx=0.11, dy=0.1, bc=(2, 2))
+    qs2[[1, 2]] = qs2[[2, 1]]
+    assert np.abs(qs2.transpose(0, 2, 1) - qs).max() <= 1e-14 * np.abs(qx).max()
+
+
+@pytest.mark.parametrize("solver", ["swe", "euler"])
+def test_split_periodic_conservation(solver):
+    q = swe_state(20, 17, seed=4) if solver == "swe" else euler_state(20, 17, seed=4)
+    qn, _ = split_step(solver, q, 0.01, bc=(2, 2))
+    for m in range(q.shape[0]):
+        assert abs(qn[m].sum() - q[m].sum()) <= 1e-13 * np.abs(q[m]).sum()
+
+
+def test_split_first_order_is_two_godunov_sweeps():
+    """order 1: donor-cell Godunov along rows, then along the columns of the intermediate state
+    (C oracle against the torch restatement built from whole-array 1D sweeps)."""
+    mx, my = 10, 9
+    q = swe_state(mx, my, seed=8)
+    d = corc.make_desc("swe", mx, my, 0.1, 0.1, grav=G, order=1, transverse_waves=0)
+    c = corc.make_ctrl(bc_lower=(1, 1), bc_upper=(1, 1), dimensional_split=True)
+    qs, cfl = corc.step_hyperbolic(d, c, q, 0.01)
+    # the same by the torch restatement built from 1D slices
+    qt, ct = ts.step_split(torch.tensor(q), 0.01, 0.1, 0.1, "swe", G, mthlim=(0, 0, 0), order=1)
+    assert np.abs(qs - qt.numpy()).max() <= 1e-14 * np.abs(q).max()
+    assert cfl > 0
+
+
+def test_split_evolve_driver():
+    """evolve_to_time with dimensional splitting: accepted steps, rejection of the dt_initial = 0.1
+    attempt and dt recurrence, re-driven step by step from Python."""
+    q, dx, dy = problems.radial_dam_break(40, 40)
+    d = corc.make_desc("swe", 40, 40, dx, dy, grav=G)
+    c = corc.make_ctrl(dimensional_split=True)
+    qe, st = corc.evolve(d, c, q, nsteps=15)
+    qq, dt, t, nacc = q.copy(), 0.1, 0.0, 0
+    while nacc < 15:
+        qn, cfl = corc.step_hyperbolic(d, c, qq, dt)
+        if cfl <= 1.0:
+            qq, t, nacc = qn, t + dt, nacc + 1
+        dt = min(1e99, dt * 0.9 / cfl)
+    assert st["numsteps"] == 15 and st["rejected"] >= 1
+    assert np.array_equal(qe, qq)
+    assert abs(st["t"] - t) <= 1e-15 * t
