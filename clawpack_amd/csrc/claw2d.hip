@@ -204,7 +204,7 @@ template <int NEQ, class T> __device__ __forceinline__ void yswap(T v[NEQ]) {
 // from qin to qout.  Shared by the single-grid and the batched kernel.  The CFL speeds and the
 // bad-state flag of the item are reduced over the wavefront and folded into *ctrl atomically.
 // LIM > 0: every wave uses limiter LIM (compile-time); LIM == 0: per-wave ids from a.mthlim.
-template <class T, class P, int LIM>
+template <class T, class P, int LIM, bool SPLIT = false>
 __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __restrict__ qin,
                                            T* __restrict__ qout, const int mx, const int my,
                                            const int x0, const int y0, const int y1, const double dt,
@@ -286,77 +286,95 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       for (int k = 0; k < NC; ++k) pl[k] = shfl(pk[k], addrL);
     }
 
-    // ---- y-interface R (between rows R-1 and R), in-lane
+    // ---- y-sweep part of row R: the y-interface between rows R-1 and R (in-lane), then the
+    // y-interface one row lower, whose limiter needed it.  cY is the cell data the y-sweeps see:
+    // q^n for the unsplit step, the x-swept q* for the dimensionally split one.
     Iface fynew;
-    if (R >= y0 - 1) {
-      P::template rpn2<1>(cprev, cR, a.par, fynew);
-      if (ycfl_col && R >= 0 && R <= my) smy = nmax(smy, P::smax(fynew));
-    }
+    auto y_part = [&](const Cell& cY) {
+      // ---- y-interface R (between rows R-1 and R), in-lane
+      if (R >= y0 - 1) {
+        P::template rpn2<1>(cprev, cY, a.par, fynew);
+        // split step: the y-speeds come from q*, which a rejected attempt (too large a dt) can
+        // leave with negative depths; as in the reference's max, NaN speeds are skipped (a NaN
+        // in an accepted state is still caught by the stored-cell check)
+        if ((SPLIT ? own_col : ycfl_col) && R >= 0 && R <= my)
+          smy = SPLIT ? vmax(smy, P::smax(fynew)) : nmax(smy, P::smax(fynew));
+      }
 
-    // ---- finish y-interface R-1 (between rows R-2 and R-1)
-    if (R >= y0) {
-      T lp[NL], ln[NL], mR[NW];
-      P::lim(fyprev, lp);
-      P::lim(fynew, ln);
-      P::mdot(lp, ln, mR);
-      if (R >= y0 + 1) {
-        T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-        finish_iface<T, P, LIM>(fyprev, yal2, ymprev, fynew.al, mR, dtdy, dtdx, a, LX, LG1, LG2,
-                                RX, RG1, RG2);
-        yswap<NEQ>(LX); yswap<NEQ>(LG1); yswap<NEQ>(LG2);
-        yswap<NEQ>(RX); yswap<NEQ>(RG1); yswap<NEQ>(RG2);
-        if constexpr (kImmY) {
-          // both cells' shares go into their accumulators now (rows R-2 and R-1)
-#pragma unroll
-          for (int m = 0; m < NEQ; ++m) {
-            const T l2 = shfl(LG2[m], addrL), l1 = shfl(LG1[m], addrR);
-            const T r2 = shfl(RG2[m], addrL), r1 = shfl(RG1[m], addrR);
-            acc0[m] += ((LX[m] + LG1[m]) - LG2[m]) + (l2 - l1);
-            acc1[m] += ((RX[m] + RG1[m]) - RG2[m]) + (r2 - r1);
+      // ---- finish y-interface R-1 (between rows R-2 and R-1)
+      if (R >= y0) {
+        T lp[NL], ln[NL], mR[NW];
+        P::lim(fyprev, lp);
+        P::lim(fynew, ln);
+        P::mdot(lp, ln, mR);
+        if (R >= y0 + 1) {
+          T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
+          finish_iface<T, P, LIM>(fyprev, yal2, ymprev, fynew.al, mR, dtdy, dtdx, a, LX, LG1, LG2,
+                                  RX, RG1, RG2);
+          yswap<NEQ>(LX); yswap<NEQ>(LG1); yswap<NEQ>(LG2);
+          yswap<NEQ>(RX); yswap<NEQ>(RG1); yswap<NEQ>(RG2);
+          if constexpr (kImmY) {
+            // both cells' shares go into their accumulators now (rows R-2 and R-1)
+  #pragma unroll
+            for (int m = 0; m < NEQ; ++m) {
+              const T l2 = shfl(LG2[m], addrL), l1 = shfl(LG1[m], addrR);
+              const T r2 = shfl(RG2[m], addrL), r1 = shfl(RG1[m], addrR);
+              acc0[m] += ((LX[m] + LG1[m]) - LG2[m]) + (l2 - l1);
+              acc1[m] += ((RX[m] + RG1[m]) - RG2[m]) + (r2 - r1);
+            }
           }
-        }
-        if (R >= y0 + 2) {
-          // cell (c, R-2) has all its y-contributions
+          if (R >= y0 + 2) {
+            // cell (c, R-2) has all its y-contributions
+            if constexpr (!kImmY) {
+  #pragma unroll
+              for (int m = 0; m < NEQ; ++m) {
+                const T Y = yX[m] + LX[m];
+                const T G1 = yG1[m] + LG1[m];
+                const T G2 = yG2[m] + LG2[m];
+                const T g2l = shfl(G2, addrL);
+                const T g1r = shfl(G1, addrR);
+                acc0[m] += ((Y + G1) - G2) + (g2l - g1r);
+              }
+            }
+            if (own_col) {
+              T qn[NEQ];
+              const size_t off = (size_t)(R - 2) * (size_t)mx + (size_t)c;
+  #pragma unroll
+              for (int m = 0; m < NEQ; ++m) {
+                qn[m] = qm2[m] + acc0[m];
+                qout[(size_t)m * plane + off] = qn[m];
+              }
+              bad |= P::bad(qn);
+            }
+          }
           if constexpr (!kImmY) {
-#pragma unroll
-            for (int m = 0; m < NEQ; ++m) {
-              const T Y = yX[m] + LX[m];
-              const T G1 = yG1[m] + LG1[m];
-              const T G2 = yG2[m] + LG2[m];
-              const T g2l = shfl(G2, addrL);
-              const T g1r = shfl(G1, addrR);
-              acc0[m] += ((Y + G1) - G2) + (g2l - g1r);
-            }
-          }
-          if (own_col) {
-            T qn[NEQ];
-            const size_t off = (size_t)(R - 2) * (size_t)mx + (size_t)c;
-#pragma unroll
-            for (int m = 0; m < NEQ; ++m) {
-              qn[m] = qm2[m] + acc0[m];
-              qout[(size_t)m * plane + off] = qn[m];
-            }
-            bad |= P::bad(qn);
+  #pragma unroll
+            for (int m = 0; m < NEQ; ++m) { yX[m] = RX[m]; yG1[m] = RG1[m]; yG2[m] = RG2[m]; }
           }
         }
-        if constexpr (!kImmY) {
-#pragma unroll
-          for (int m = 0; m < NEQ; ++m) { yX[m] = RX[m]; yG1[m] = RG1[m]; yG2[m] = RG2[m]; }
+  #pragma unroll
+        for (int p = 0; p < NW; ++p) {
+          ymprev[p] = mR[p];
+          yal2[p] = fyprev.al[p];
         }
       }
-#pragma unroll
-      for (int p = 0; p < NW; ++p) {
-        ymprev[p] = mR[p];
-        yal2[p] = fyprev.al[p];
-      }
-    }
 
-    // ---- x-interfaces of row R (the left cells were requested before the y work)
-    if (R >= y0 - 1 && R <= y1) {
+
+    };
+
+    if constexpr (!SPLIT) y_part(cR);
+
+    // ---- x-interfaces of row R (the left cells were requested before the y work).  Split step:
+    // every marched row (the y-sweeps need q* of the halo rows too); ghost rows are x-swept from
+    // their mapped source rows, which equals refilling the ghosts of q* for extrap, periodic and
+    // wall conditions.
+    Cell cY = cR;
+    if (SPLIT || (R >= y0 - 1 && R <= y1)) {
       const Cell cL = P::unpack(pl);
       Iface fxi;
       P::template rpn2<0>(cL, cR, a.par, fxi);
-      if (xcfl_col && R >= -1 && R <= my) smx = nmax(smx, P::smax(fxi));
+      if (SPLIT ? (xcfl_col && R >= 0 && R < my) : (xcfl_col && R >= -1 && R <= my))
+        smx = nmax(smx, P::smax(fxi));
       // limiter partners: both neighbours' wave data in one exchange round; the left pair's
       // eigenvector products are recomputed here (the same operands in the same order as on the
       // left lane, so the same bits)
@@ -376,16 +394,25 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       }
       T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
       finish_iface<T, P, LIM>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2);
+      if constexpr (SPLIT) {
+        // q* of row R: the x-sweep only (no transverse terms); valid on the owned columns
+        T qs[NEQ];
 #pragma unroll
-      for (int m = 0; m < NEQ; ++m) {
-        const T X = RX[m] + shfl(LX[m], addrR);
-        const T G1 = RG1[m] + shfl(LG1[m], addrR);
-        const T G2 = RG2[m] + shfl(LG2[m], addrR);
-        acc1[m] -= G1;
-        acc2[m] += (X + G1) - G2;
-        acc3[m] += G2;
+        for (int m = 0; m < NEQ; ++m) qs[m] = qR[m] + (RX[m] + shfl(LX[m], addrR));
+        cY = P::prep(qs, a.par);
+      } else {
+#pragma unroll
+        for (int m = 0; m < NEQ; ++m) {
+          const T X = RX[m] + shfl(LX[m], addrR);
+          const T G1 = RG1[m] + shfl(LG1[m], addrR);
+          const T G2 = RG2[m] + shfl(LG2[m], addrR);
+          acc1[m] -= G1;
+          acc2[m] += (X + G1) - G2;
+          acc3[m] += G2;
+        }
       }
     }
+    if constexpr (SPLIT) y_part(cY);
 
     // ---- roll the column window
 #pragma unroll
@@ -396,7 +423,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       acc2[m] = acc3[m];
       acc3[m] = T(0);
     }
-    cprev = cR;
+    cprev = cY;
     fyprev = fynew;
   }
 
@@ -507,7 +534,7 @@ template <class T> struct MinWaves {
   static constexpr int value = sizeof(T) == 8 ? CLAW_MIN_WAVES_F64 : CLAW_MIN_WAVES_F32;
 };
 
-template <class T, class P, int LIM>
+template <class T, class P, int LIM, bool SPLIT>
 __global__ void __launch_bounds__(kLanes* kWPB) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::value)))
 step2_fused(StepArgs<T, P> a) {
   double dt = a.dt_fixed;
@@ -524,7 +551,7 @@ step2_fused(StepArgs<T, P> a) {
   const int strip = wave % a.nstrips, seg = wave / a.nstrips;
   const int x0 = strip * kOwn, y0 = seg * a.seg_h;
   const int y1 = min(y0 + a.seg_h, a.my);
-  march_item<T, P, LIM>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
+  march_item<T, P, LIM, SPLIT>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
   if (a.fuse) {
     // The last wavefront to finish closes the step: its speed maxima are then complete.  The
     // wait for this wavefront's outstanding memory operations (its max/or atomics are acknowledged
@@ -730,14 +757,15 @@ template <class T> static typename EulerP<T>::Par euler_par(const claw2d_desc* d
   return p;
 }
 
-template <class T, class P>
+template <class T, class P, bool SPLIT>
 static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q0, void* q1,
                        claw2d_ctrl* ctrl, int use_ctrl, double dt, hipStream_t s) {
   bool same = true;
   for (int p = 1; p < P::NW; ++p) same = same && d->mthlim[p] == d->mthlim[0];
   const int lim = same ? d->mthlim[0] : -1;
-  const void* kern = lim == 4 ? (const void*)step2_fused<T, P, 4>
-                   : (lim == 1 ? (const void*)step2_fused<T, P, 1> : (const void*)step2_fused<T, P, 0>);
+  const void* kern = lim == 4 ? (const void*)step2_fused<T, P, 4, SPLIT>
+                   : (lim == 1 ? (const void*)step2_fused<T, P, 1, SPLIT>
+                               : (const void*)step2_fused<T, P, 0, SPLIT>);
   static thread_local long cached[3] = {-1, -1, -1};
   const int slot = lim == 4 ? 0 : (lim == 1 ? 1 : 2);
   if (cached[slot] < 0) cached[slot] = resident_waves(kern);
@@ -753,28 +781,30 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   a.cfl_max = d->cfl_max;
   a.cfl_desired = d->cfl_desired;
   a.dt_max = d->dt_max;
+  if (SPLIT) a.tw = 0;  // dimensional splitting propagates no transverse waves (method(3) < 0)
   const int waves = a.nstrips * a.nsegs;
   a.total_waves = waves;
   const int blocks = (waves + kWPB - 1) / kWPB;
   if (lim == 4)
-    hipLaunchKernelGGL((step2_fused<T, P, 4>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
+    hipLaunchKernelGGL((step2_fused<T, P, 4, SPLIT>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
   else if (lim == 1)
-    hipLaunchKernelGGL((step2_fused<T, P, 1>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
+    hipLaunchKernelGGL((step2_fused<T, P, 1, SPLIT>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
   else
-    hipLaunchKernelGGL((step2_fused<T, P, 0>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
+    hipLaunchKernelGGL((step2_fused<T, P, 0, SPLIT>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : hip_fail(e, "step2_fused launch");
 }
 
+template <bool SPLIT>
 static int dispatch_step(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, int use_ctrl,
                          double dt, hipStream_t s) {
   const bool swe = d->solver == CLAW_SOLVER_SHALLOW_ROE_EFIX;
   if (d->dtype == CLAW_FP64) {
-    return swe ? launch_step<double, SweP<double>>(d, swe_par<double>(d), q0, q1, ctrl, use_ctrl, dt, s)
-               : launch_step<double, EulerP<double>>(d, euler_par<double>(d), q0, q1, ctrl, use_ctrl, dt, s);
+    return swe ? launch_step<double, SweP<double>, SPLIT>(d, swe_par<double>(d), q0, q1, ctrl, use_ctrl, dt, s)
+               : launch_step<double, EulerP<double>, SPLIT>(d, euler_par<double>(d), q0, q1, ctrl, use_ctrl, dt, s);
   }
-  return swe ? launch_step<float, SweP<float>>(d, swe_par<float>(d), q0, q1, ctrl, use_ctrl, dt, s)
-             : launch_step<float, EulerP<float>>(d, euler_par<float>(d), q0, q1, ctrl, use_ctrl, dt, s);
+  return swe ? launch_step<float, SweP<float>, SPLIT>(d, swe_par<float>(d), q0, q1, ctrl, use_ctrl, dt, s)
+             : launch_step<float, EulerP<float>, SPLIT>(d, euler_par<float>(d), q0, q1, ctrl, use_ctrl, dt, s);
 }
 
 template <class T, class P>
@@ -896,7 +926,7 @@ int claw2d_step(const claw2d_desc* d, const void* q_in, void* q_out, double dt, 
   if (q_in == q_out) return fail(-23, "q_in and q_out must differ");
   if (!(dt > 0)) return fail(-21, "dt must be positive");
   hipStream_t s = (hipStream_t)stream;
-  return dispatch_step(d, const_cast<void*>(q_in), q_out, ctrl, 0, dt, s);
+  return dispatch_step<false>(d, const_cast<void*>(q_in), q_out, ctrl, 0, dt, s);
 }
 
 int claw2d_advance(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, long long attempts,
@@ -907,7 +937,30 @@ int claw2d_advance(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, 
   if (q0 == q1) return fail(-23, "q0 and q1 must differ");
   hipStream_t s = (hipStream_t)stream;
   for (long long k = 0; k < attempts; ++k) {
-    rc = dispatch_step(d, q0, q1, ctrl, 1, 0.0, s);
+    rc = dispatch_step<false>(d, q0, q1, ctrl, 1, 0.0, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int claw2d_step_ds(const claw2d_desc* d, const void* q_in, void* q_out, double dt, claw2d_ctrl* ctrl,
+                   void* stream) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!q_in || !q_out || !ctrl) return fail(-22, "null buffer");
+  if (q_in == q_out) return fail(-23, "q_in and q_out must differ");
+  if (!(dt > 0)) return fail(-21, "dt must be positive");
+  return dispatch_step<true>(d, const_cast<void*>(q_in), q_out, ctrl, 0, dt, (hipStream_t)stream);
+}
+
+int claw2d_advance_ds(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, long long attempts,
+                      void* stream) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!q0 || !q1 || !ctrl) return fail(-22, "null buffer");
+  if (q0 == q1) return fail(-23, "q0 and q1 must differ");
+  for (long long k = 0; k < attempts; ++k) {
+    rc = dispatch_step<true>(d, q0, q1, ctrl, 1, 0.0, (hipStream_t)stream);
     if (rc) return rc;
   }
   return 0;

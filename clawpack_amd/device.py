@@ -42,9 +42,10 @@ def make_desc(solver_id, dtype, mx, my, dx, dy, num_eqn, num_waves, order=2, tra
 
 
 class DeviceGrid:
-    """One grid on one GPU: q ping-pong pair, control block, mass-sum workspace."""
+    """One grid on one GPU: q ping-pong pair, control block, mass-sum workspace.  split=True
+    steps with dimensional splitting (step2ds) instead of the unsplit step2."""
 
-    def __init__(self, desc, device="cuda"):
+    def __init__(self, desc, device="cuda", split=False):
         self.lib = _lib.load()
         check(self.lib.claw2d_validate(C.byref(desc)), "claw2d_validate")
         if not torch.cuda.is_available():
@@ -61,6 +62,13 @@ class DeviceGrid:
         self.mass_ws = torch.zeros((ws + 7) // 8, dtype=torch.float64, device=self.device)
         self.mass_out = torch.zeros(1, dtype=torch.float64, device=self.device)
         self.cur = 0
+        self.split = bool(split)
+
+    def _step_fn(self):
+        return self.lib.claw2d_step_ds if self.split else self.lib.claw2d_step
+
+    def _advance_fn(self):
+        return self.lib.claw2d_advance_ds if self.split else self.lib.claw2d_advance
 
     # ---- control block ------------------------------------------------------------------
     @property
@@ -109,9 +117,9 @@ class DeviceGrid:
         """One f2py-style step2 with fixed dt (q[cur] -> q[other]); returns the CFL number.
         The control block's smax accumulators must be zero (init_ctrl leaves them zero)."""
         src, dst = self.q[self.cur], self.q[1 - self.cur]
-        check(self.lib.claw2d_step(C.byref(self.desc), C.c_void_p(src.data_ptr()),
-                                   C.c_void_p(dst.data_ptr()), float(dt), self.ctrl_ptr,
-                                   _stream_handle(stream)), "claw2d_step")
+        check(self._step_fn()(C.byref(self.desc), C.c_void_p(src.data_ptr()),
+                              C.c_void_p(dst.data_ptr()), float(dt), self.ctrl_ptr,
+                              _stream_handle(stream)), "claw2d_step")
         self.cur = 1 - self.cur
         c = self.read_ctrl()
         return c.cfl, c
@@ -119,16 +127,16 @@ class DeviceGrid:
     def enqueue_step(self, dt, stream=None):
         """Like step() but without reading the control block back (no host synchronisation)."""
         src, dst = self.q[self.cur], self.q[1 - self.cur]
-        check(self.lib.claw2d_step(C.byref(self.desc), C.c_void_p(src.data_ptr()),
-                                   C.c_void_p(dst.data_ptr()), float(dt), self.ctrl_ptr,
-                                   _stream_handle(stream)), "claw2d_step")
+        check(self._step_fn()(C.byref(self.desc), C.c_void_p(src.data_ptr()),
+                              C.c_void_p(dst.data_ptr()), float(dt), self.ctrl_ptr,
+                              _stream_handle(stream)), "claw2d_step")
         self.cur = 1 - self.cur
 
     def advance(self, attempts, stream=None):
         """Enqueue `attempts` evolve_to_time attempts (device-side accept/reject and dt)."""
-        check(self.lib.claw2d_advance(C.byref(self.desc), C.c_void_p(self.q[0].data_ptr()),
-                                      C.c_void_p(self.q[1].data_ptr()), self.ctrl_ptr,
-                                      int(attempts), _stream_handle(stream)), "claw2d_advance")
+        check(self._advance_fn()(C.byref(self.desc), C.c_void_p(self.q[0].data_ptr()),
+                                 C.c_void_p(self.q[1].data_ptr()), self.ctrl_ptr,
+                                 int(attempts), _stream_handle(stream)), "claw2d_advance")
 
     def sync_cur(self, c=None):
         c = self.read_ctrl() if c is None else c

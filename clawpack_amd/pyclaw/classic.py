@@ -20,15 +20,16 @@ _KERNEL_LANGUAGES = ("HIP",)
 class ClawSolver2D(Solver):
     """Two-dimensional classic (wave-propagation) solver.
 
-    Supported on the HIP path: dimensional_split = False, order 1 or 2, transverse_waves 0/1/2,
-    limiters 0-5 per wave, BC extrap/periodic/wall, the Riemann plugins
-    shallow_roe_with_efix_2D and euler_4wave_2D, solver.precision 'fp64' (default) or 'fp32'.
+    Supported on the HIP path: dimensional_split True (step2ds, Godunov x-then-y splitting, the
+    default) or False (unsplit step2 with transverse_waves 0/1/2), order 1 or 2, limiters 0-5 per
+    wave, BC extrap/periodic/wall, the Riemann plugins shallow_roe_with_efix_2D and
+    euler_4wave_2D, solver.precision 'fp64' (default) or 'fp32'.
     """
 
     def __init__(self, riemann_solver=None, claw_package=None):
         super().__init__(riemann_solver, claw_package)
         self.num_dim = 2
-        self.dimensional_split = True   # pyclaw's default; the HIP path needs False
+        self.dimensional_split = True   # pyclaw's default: step2ds
         self.transverse_waves = 1
         self.order = 2
         self.limiters = tvd.minmod
@@ -59,9 +60,6 @@ class ClawSolver2D(Solver):
         if self.rp is None or not hasattr(self.rp, "kernel_id"):
             raise NotImplementedError("Riemann solver not available on the HIP path: use "
                                       "riemann.shallow_roe_with_efix_2D or riemann.euler_4wave_2D")
-        if self.dimensional_split:
-            raise NotImplementedError("dimensional_split=True (step2ds) is not implemented on the HIP "
-                                      "path; set solver.dimensional_split = False (unsplit step2)")
         if self.fwave:
             raise NotImplementedError("fwave=True is not supported on the HIP path")
         if state.num_aux > 0 or state.index_capa >= 0:
@@ -123,11 +121,13 @@ class ClawSolver2D(Solver):
             self._desc_key = key
         else:
             self._grid.desc = desc
+        self._grid.split = bool(self.dimensional_split)
         return self._grid
 
     # ----------------------------------------------------------------------------- steps ---
     def step_hyperbolic(self, solution):
-        """One step2 with the current dt (no accept/reject), as ClawSolver2D.step_hyperbolic."""
+        """One step (step2, or step2ds when dimensional_split) with the current dt, no
+        accept/reject, as ClawSolver2D.step_hyperbolic."""
         if not self._is_set_up:
             self.setup(solution)
         state = solution.state

@@ -110,6 +110,18 @@ int claw2d_step(const claw2d_desc* d, const void* q_in_dev, void* q_out_dev, dou
 int claw2d_advance(const claw2d_desc* d, void* q0_dev, void* q1_dev, claw2d_ctrl* ctrl_dev,
                    long long attempts, void* stream);
 
+/* Dimensionally split (Godunov, x then y) versions of claw2d_step / claw2d_advance: replace
+ * classic2.step2ds (upstream classic/step2ds.f90), which ClawSolver2D.step_hyperbolic calls when
+ * solver.dimensional_split is True (pyclaw's default in 2D).  The x-sweeps over all rows, a fresh
+ * ghost fill of the intermediate state q* and the y-sweeps over all columns; no transverse
+ * propagation (transverse_waves is ignored); cfl = max of the two sweeps' CFL numbers.  One kernel
+ * per attempt: q* is formed on the fly (each marched row is x-swept before its y-interfaces), so it
+ * never goes to memory. */
+int claw2d_step_ds(const claw2d_desc* d, const void* q_in_dev, void* q_out_dev, double dt,
+                   claw2d_ctrl* ctrl_dev, void* stream);
+int claw2d_advance_ds(const claw2d_desc* d, void* q0_dev, void* q1_dev, claw2d_ctrl* ctrl_dev,
+                      long long attempts, void* stream);
+
 /* Bytes of the workspace claw2d_mass_sum needs. */
 size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc* d);
 

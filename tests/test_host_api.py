@@ -48,7 +48,6 @@ def test_defaults_follow_pyclaw():
 @pytest.mark.parametrize("attr,value,exc", [
     ("kernel_language", "Fortran", NotImplementedError),
     ("kernel_language", "Python", NotImplementedError),
-    ("dimensional_split", True, NotImplementedError),
     ("fwave", True, NotImplementedError),
     ("order", 3, ValueError),
     ("precision", "fp16", ValueError),
