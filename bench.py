@@ -50,6 +50,7 @@ def parse():
     p.add_argument("--batch-grids", type=int, default=512)
     p.add_argument("--batch-steps", type=int, default=200)
     p.add_argument("--seq-steps", type=int, default=10)
+    p.add_argument("--no-split", action="store_true")
     p.add_argument("--no-euler", action="store_true")
     p.add_argument("--euler-n", type=int, default=2048)
     p.add_argument("--euler-steps", type=int, default=500)
@@ -81,7 +82,7 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": n * 8, "repeats": 5}), flush=True)
 
 
-def run_gpu(n, dtype_id, steps, warmup, seed):
+def run_gpu(n, dtype_id, steps, warmup, seed, split=False):
     import torch
     from clawpack_amd import problems
     from clawpack_amd.device import DeviceGrid, make_desc
@@ -89,7 +90,7 @@ def run_gpu(n, dtype_id, steps, warmup, seed):
     q, dx, dy = problems.gaussian_humps(n, n, seed)
     d = make_desc(0, dtype_id, n, n, dx, dy, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
                   bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
-    g = DeviceGrid(d, "cuda:0")
+    g = DeviceGrid(d, "cuda:0", split=split)
     g.upload(q)
     del q
     # dt from the initial wave speed, as after pyclaw's first accepted step
@@ -385,6 +386,16 @@ def main():
         out["fp32"] = {"value": v32, "ms_per_step": r32["ms"] / r32["accepted"],
                        "ratio_vs_fp64": v32 / value,
                        "roofline_frac": 24.0 * v32 / 1e9 / HBM_PEAK_GBS}
+    if not a.no_split:
+        sp = {}
+        for dt_id, name, bpc in ((0, "fp64", 48.0), (1, "fp32", 24.0)):
+            r = run_gpu(a.n, dt_id, a.steps, a.warmup, a.seed, split=True)
+            sp[name] = {"cell_updates_per_s": r["cells"] * r["accepted"] / (r["ms"] * 1e-3),
+                        "ms_per_step": r["ms"] / r["accepted"],
+                        "roofline_frac": bpc * r["cells"] * r["attempts"] / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        sp["workload"] = "same as the headline line with dimensional_split=True (step2ds, one pass per step)"
+        out["split"] = sp
+        progress("split done")
     if not a.no_batch:
         out["batch"] = {"grids": a.batch_grids, "fp64": run_batch(a.batch_grids, 0, a.batch_steps, a.seq_steps, a.seed),
                         "fp32": run_batch(a.batch_grids, 1, a.batch_steps, a.seq_steps, a.seed)}

@@ -87,6 +87,9 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    # torch first: the library then binds to the HIP runtime torch already loaded (same soname),
+    # instead of loading the system one ahead of torch's and leaving torch without a device
+    import torch  # noqa: F401
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             "libclawpack_amd.so is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
