@@ -1,6 +1,6 @@
 """Controller (upstream pyclaw/src/pyclaw/controller.py): output-time loop around
-solver.evolve_to_time.  Frames are kept in memory (keep_copy); writing frame files is not
-part of this package."""
+solver.evolve_to_time.  Frames are written to outdir in the ascii format (output_format='ascii',
+the default; None writes nothing) and/or kept in memory (keep_copy)."""
 import copy
 
 import numpy as np
@@ -15,7 +15,11 @@ class Controller:
         self.output_style = 1
         self.out_times = None
         self.keep_copy = False
-        self.output_format = None
+        self.output_format = "ascii"
+        self.output_file_prefix = None
+        self.output_options = {}
+        self.write_aux_init = False
+        self.write_aux_always = False
         self.outdir = "./_output"
         self.frames = []
         self.verbosity = 0
@@ -31,11 +35,9 @@ class Controller:
     def run(self):
         if self.solution is None or self.solver is None:
             raise Exception("Controller.run needs solution and solver")
-        if self.output_format is not None:
-            raise NotImplementedError("frame output is not part of clawpack_amd; set output_format=None")
         self.solver.setup(self.solution)
-        if self.keep_copy:
-            self.frames.append(copy.deepcopy(self.solution))
+        frame = 0
+        self._output(frame, self.write_aux_init or self.write_aux_always)
         totals = {"cflmax": 0.0, "dtmin": None, "dtmax": None, "numsteps": 0}
         for t in self._times()[1:]:
             st = self.solver.evolve_to_time(self.solution, t)
@@ -43,6 +45,13 @@ class Controller:
             totals["dtmin"] = st["dtmin"] if totals["dtmin"] is None else min(totals["dtmin"], st["dtmin"])
             totals["dtmax"] = st["dtmax"] if totals["dtmax"] is None else max(totals["dtmax"], st["dtmax"])
             totals["numsteps"] += st["numsteps"]
-            if self.keep_copy:
-                self.frames.append(copy.deepcopy(self.solution))
+            frame += 1
+            self._output(frame, self.write_aux_always)
         return totals
+
+    def _output(self, frame, write_aux):
+        if self.output_format is not None:
+            self.solution.write(frame, self.outdir, self.output_format, self.output_file_prefix,
+                                write_aux, self.output_options)
+        if self.keep_copy:
+            self.frames.append(copy.deepcopy(self.solution))

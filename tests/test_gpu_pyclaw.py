@@ -42,6 +42,7 @@ def test_controller_dam_break_matches_oracle():
     claw.solution, claw.solver = sol, solver
     claw.tfinal, claw.num_output_times = 0.5, 5
     claw.keep_copy = True
+    claw.output_format = None
     status = claw.run()
     assert len(claw.frames) == 6 and abs(sol.t - 0.5) < 1e-15
     # the oracle driven through the same output times
@@ -166,3 +167,20 @@ def test_max_steps_raises():
     solver.dt = 0.01  # no rejected attempt: pyclaw raises only when max_steps steps were accepted
     with pytest.raises(Exception, match="Maximum number of timesteps"):
         solver.evolve_to_time(sol, 5.0)
+
+
+def test_controller_writes_ascii_frames(tmp_path):
+    """The drop-in Controller with its default output_format='ascii': frames 0..N on disk, read
+    back with Solution(frame, path=...), equal to the in-memory frames to the format's 9 digits."""
+    print("case controller ascii frames 64x64", flush=True)
+    solver, sol = setup_dam_break(64)
+    claw = pyclaw.Controller()
+    claw.solution, claw.solver = sol, solver
+    claw.tfinal, claw.num_output_times = 0.3, 3
+    claw.outdir = str(tmp_path)
+    claw.keep_copy = True
+    claw.run()
+    for k in range(4):
+        back = pyclaw.Solution(k, path=str(tmp_path))
+        assert abs(back.t - claw.frames[k].t) <= 1e-8 * max(1.0, claw.frames[k].t)
+        np.testing.assert_allclose(back.state.q, claw.frames[k].state.q, rtol=1e-8, atol=1e-12)
