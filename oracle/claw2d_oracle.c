@@ -9,6 +9,9 @@
  *   rpn2/rpt2 shallow_roe_with_efix              (riemann/src/rpn2_shallow_roe_with_efix.f90,
  *                                                 riemann/src/rpt2_shallow_roe_with_efix.f90)
  *   rpn2 euler_4wave / rpt2 euler                (riemann/src/rpn2_euler_4wave.f90, rpt2_euler.f90)
+ *   rpn2/rpt2 shallow_bathymetry_fwave           (riemann/src/rpn2_shallow_bathymetry_fwave.f90,
+ *                                                 rpt2_shallow_bathymetry_fwave.f90), with the
+ *                                                 f-wave form of flux2 (use_fwave) and aux arrays
  *   ghost fill + dt control                      (pyclaw/src/pyclaw/solver.py: _apply_bcs,
  *                                                 _bc_lower/_bc_upper, evolve_to_time)
  *
@@ -32,7 +35,7 @@
 #define ORC_MAXEQN 4
 #define ORC_MAXWAVES 4
 
-enum { ORC_SWE = 0, ORC_EULER = 1 };
+enum { ORC_SWE = 0, ORC_EULER = 1, ORC_SWE_BATHY = 2 };
 enum { BC_CUSTOM = 0, BC_EXTRAP = 1, BC_PERIODIC = 2, BC_WALL = 3 };
 
 typedef struct {
@@ -47,6 +50,9 @@ typedef struct {
     double dx, dy;
     double grav;          /* problem_data['grav']  -> cparam */
     double gamma;         /* problem_data['gamma'] -> cparam */
+    int num_aux;          /* method(7); aux(1) is the bathymetry b for ORC_SWE_BATHY */
+    int fwave;            /* solver.fwave: the Riemann solver returns f-waves Z (sum Z = flux jump) */
+    double dry_tolerance; /* problem_data['dry_tolerance'] (ORC_SWE_BATHY) */
 } orc_desc;
 
 typedef struct {
@@ -56,6 +62,7 @@ typedef struct {
     int bc_lower[2], bc_upper[2];
     int reflect_index[2]; /* 0-based q component negated at a wall, per dimension */
     int dimensional_split; /* 1: Godunov splitting (step2ds), 0: unsplit (step2) */
+    int aux_bc_lower[2], aux_bc_upper[2]; /* solver.aux_bc_lower/upper (wall mirrors, no sign flip) */
 } orc_ctrl;
 
 typedef struct {
@@ -70,10 +77,12 @@ typedef struct {
 typedef struct {
     int cap;
     double *wave, *s, *amdq, *apdq, *cqxx, *bmasdq, *bpasdq, *qadd, *fadd, *gadd, *q1d;
+    double *aux1d; /* aux(maux, slice) */
 } orc_work;
 
-static int work_alloc(orc_work *w, int nslice, int meqn, int mwaves) {
+static int work_alloc(orc_work *w, int nslice, int meqn, int mwaves, int maux) {
     w->cap = nslice;
+    w->aux1d = calloc((size_t)nslice * (maux > 0 ? maux : 1), sizeof(double));
     w->wave = calloc((size_t)nslice * meqn * mwaves, sizeof(double));
     w->s = calloc((size_t)nslice * mwaves, sizeof(double));
     w->amdq = calloc((size_t)nslice * meqn, sizeof(double));
@@ -86,12 +95,13 @@ static int work_alloc(orc_work *w, int nslice, int meqn, int mwaves) {
     w->gadd = calloc((size_t)nslice * meqn * 2, sizeof(double));
     w->q1d = calloc((size_t)nslice * meqn, sizeof(double));
     return (w->wave && w->s && w->amdq && w->apdq && w->cqxx && w->bmasdq && w->bpasdq &&
-            w->qadd && w->fadd && w->gadd && w->q1d) ? 0 : -1;
+            w->qadd && w->fadd && w->gadd && w->q1d && w->aux1d) ? 0 : -1;
 }
 
 static void work_free(orc_work *w) {
     free(w->wave); free(w->s); free(w->amdq); free(w->apdq); free(w->cqxx);
     free(w->bmasdq); free(w->bpasdq); free(w->qadd); free(w->fadd); free(w->gadd); free(w->q1d);
+    free(w->aux1d);
 }
 
 /* 1-based slice index -> 0-based storage offset */
@@ -194,6 +204,109 @@ static void rpt2_swe(int ixy, int n, int g, int meqn, const double *q1d, double 
         const double u = (Q1(mu, i - 1) / hsqrtl + Q1(mu, i) / hsqrtr) / hsq2;
         const double v = (Q1(mv, i - 1) / hsqrtl + Q1(mv, i) / hsqrtr) / hsq2;
         const double a = sqrt(grav * hbar);
+        const double as0 = A1(asdq, 0, i), asu = A1(asdq, mu, i), asv = A1(asdq, mv, i);
+        const double b1 = (0.5 / a) * ((v + a) * as0 - asv);
+        const double b2 = asu - u * as0;
+        const double b3 = (0.5 / a) * (-(v - a) * as0 + asv);
+        double wb[3][3], sb[3];
+        wb[0][0] = b1;  wb[0][mu] = b1 * u;  wb[0][mv] = b1 * (v - a);  sb[0] = v - a;
+        wb[1][0] = 0.0; wb[1][mu] = b2;      wb[1][mv] = 0.0;           sb[1] = v;
+        wb[2][0] = b3;  wb[2][mu] = b3 * u;  wb[2][mv] = b3 * (v + a);  sb[2] = v + a;
+        for (int m = 0; m < meqn; ++m) {
+            double bm = 0.0, bp = 0.0;
+            for (int mw = 0; mw < 3; ++mw) {
+                bm += fmin(sb[mw], 0.0) * wb[mw][m];
+                bp += fmax(sb[mw], 0.0) * wb[mw][m];
+            }
+            A1(bmasdq, m, i) = bm;
+            A1(bpasdq, m, i) = bp;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Shallow water over bathymetry, f-wave form.  Restates riemann/src/rpn2_shallow_bathymetry_
+ * fwave.f90 and rpt2_shallow_bathymetry_fwave.f90 (upstream, absent) from the published method
+ * (Bale, LeVeque, Mitran, Rossmanith, SIAM J. Sci. Comput. 24 (2002) 955-978, cited at
+ * CITATION.md:74-88): the flux jump minus the bathymetry source,
+ *   delta = f(q_r) - f(q_l) + (0, g hbar (b_r - b_l), 0),
+ * is split along the Roe eigenvectors (1, u-c, v), (0, 0, 1), (1, u+c, v) into f-waves Z_p with
+ * Roe speeds u-c, u, u+c; amdq/apdq take the f-waves of negative/positive speed (half of a
+ * zero-speed one each).  A cell with h <= dry_tolerance has zero velocity; an interface with
+ * both cells dry has no waves.  b = aux(1).  For a flat bottom and no entropy-fix branch the
+ * f-waves are s_p times the Roe waves of rpn2_swe (the Roe property), a lake at rest
+ * (u = v = 0, h + b constant) gives delta = 0.
+ * ---------------------------------------------------------------------------------------- */
+/* Roe averages of a bathymetry interface; 0 if both cells are dry. */
+static int roe_bathy(double hl, double hul, double hvl, double hr, double hur, double hvr,
+                     double grav, double tol, double *ul, double *vl, double *ur, double *vr,
+                     double *uhat, double *vhat, double *chat) {
+    if (!(hl > tol) && !(hr > tol)) return 0;
+    *ul = hl > tol ? hul / hl : 0.0;
+    *vl = hl > tol ? hvl / hl : 0.0;
+    *ur = hr > tol ? hur / hr : 0.0;
+    *vr = hr > tol ? hvr / hr : 0.0;
+    const double sql = sqrt(hl), sqr = sqrt(hr);
+    *uhat = (sql * *ul + sqr * *ur) / (sql + sqr);
+    *vhat = (sql * *vl + sqr * *vr) / (sql + sqr);
+    *chat = sqrt(grav * (0.5 * (hl + hr)));
+    return 1;
+}
+
+static void rpn2_bathy(int ixy, int n, int g, int meqn, int mwaves, const double *q1d,
+                       const double *aux1d, int maux, double grav, double tol, double *wave,
+                       double *s, double *amdq, double *apdq) {
+    const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
+    for (int i = 2 - g; i <= n + g; ++i) {
+        for (int m = 0; m < meqn; ++m) {
+            A1(amdq, m, i) = 0.0;
+            A1(apdq, m, i) = 0.0;
+            for (int mw = 0; mw < mwaves; ++mw) WV(m, mw, i) = 0.0;
+        }
+        for (int mw = 0; mw < mwaves; ++mw) SP(mw, i) = 0.0;
+        const double hl = Q1(0, i - 1), hr = Q1(0, i);
+        double ul, vl, ur, vr, u, v, c;
+        if (!roe_bathy(hl, Q1(mu, i - 1), Q1(mv, i - 1), hr, Q1(mu, i), Q1(mv, i), grav, tol,
+                       &ul, &vl, &ur, &vr, &u, &v, &c))
+            continue;
+        const double bl = aux1d[maux * SL(i - 1)], br = aux1d[maux * SL(i)];
+        const double d1 = hr * ur - hl * ul;
+        const double d2 = (hr * ur * ur + 0.5 * grav * hr * hr) - (hl * ul * ul + 0.5 * grav * hl * hl)
+                          + grav * (0.5 * (hl + hr)) * (br - bl);
+        const double d3 = hr * ur * vr - hl * ul * vl;
+        const double s1 = u - c, s3 = u + c;
+        const double b1 = (s3 * d1 - d2) / (s3 - s1);
+        const double b3 = (d2 - s1 * d1) / (s3 - s1);
+        const double b2 = d3 - v * d1;
+        WV(0, 0, i) = b1;  WV(mu, 0, i) = b1 * s1; WV(mv, 0, i) = b1 * v;  SP(0, i) = s1;
+        WV(0, 1, i) = 0.0; WV(mu, 1, i) = 0.0;     WV(mv, 1, i) = b2;      SP(1, i) = u;
+        WV(0, 2, i) = b3;  WV(mu, 2, i) = b3 * s3; WV(mv, 2, i) = b3 * v;  SP(2, i) = s3;
+        for (int mw = 0; mw < mwaves; ++mw)
+            for (int m = 0; m < meqn; ++m) {
+                if (SP(mw, i) < 0.0) {
+                    A1(amdq, m, i) += WV(m, mw, i);
+                } else if (SP(mw, i) > 0.0) {
+                    A1(apdq, m, i) += WV(m, mw, i);
+                } else {
+                    A1(amdq, m, i) += 0.5 * WV(m, mw, i);
+                    A1(apdq, m, i) += 0.5 * WV(m, mw, i);
+                }
+            }
+    }
+}
+
+/* Transverse split of asdq at a bathymetry interface: the rpt2_swe eigen-split with the Roe
+ * averages of roe_bathy (zero velocity in dry cells); nothing moves across two dry cells. */
+static void rpt2_bathy(int ixy, int n, int g, int meqn, const double *q1d, double grav, double tol,
+                       const double *asdq, double *bmasdq, double *bpasdq) {
+    const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
+    for (int i = 2 - g; i <= n + g; ++i) {
+        double ul, vl, ur, vr, u, v, a;
+        if (!roe_bathy(Q1(0, i - 1), Q1(mu, i - 1), Q1(mv, i - 1), Q1(0, i), Q1(mu, i), Q1(mv, i),
+                       grav, tol, &ul, &vl, &ur, &vr, &u, &v, &a)) {
+            for (int m = 0; m < meqn; ++m) A1(bmasdq, m, i) = A1(bpasdq, m, i) = 0.0;
+            continue;
+        }
         const double as0 = A1(asdq, 0, i), asu = A1(asdq, mu, i), asv = A1(asdq, mv, i);
         const double b1 = (0.5 / a) * ((v + a) * as0 - asv);
         const double b2 = asu - u * as0;
@@ -420,6 +533,9 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
 
     if (d->solver == ORC_SWE)
         rpn2_swe(ixy, n, g, meqn, mwaves, q1d, d->grav, wave, s, amdq, apdq);
+    else if (d->solver == ORC_SWE_BATHY)
+        rpn2_bathy(ixy, n, g, meqn, mwaves, q1d, w->aux1d, d->num_aux, d->grav, d->dry_tolerance,
+                   wave, s, amdq, apdq);
     else
         rpn2_euler(ixy, n, g, meqn, mwaves, q1d, d->gamma, wave, s, amdq, apdq);
 
@@ -446,8 +562,11 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
             const double dtdxave = 0.5 * (dtdx + dtdx);
             for (int m = 0; m < meqn; ++m) {
                 double cq = 0.0;
-                for (int mw = 0; mw < mwaves; ++mw)
-                    cq += fabs(SP(mw, i)) * (1.0 - fabs(SP(mw, i)) * dtdxave) * WV(m, mw, i);
+                for (int mw = 0; mw < mwaves; ++mw) {
+                    /* f-waves already carry the speed: sign(s) in place of |s| (dsign(1, s)) */
+                    const double abs_sign = d->fwave ? copysign(1.0, SP(mw, i)) : fabs(SP(mw, i));
+                    cq += abs_sign * (1.0 - fabs(SP(mw, i)) * dtdxave) * WV(m, mw, i);
+                }
                 A1(cqxx, m, i) = cq;
                 A1(fadd, m, i) += 0.5 * cq;
             }
@@ -463,6 +582,8 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
 
     if (d->solver == ORC_SWE)
         rpt2_swe(ixy, n, g, meqn, q1d, d->grav, amdq, bmasdq, bpasdq);
+    else if (d->solver == ORC_SWE_BATHY)
+        rpt2_bathy(ixy, n, g, meqn, q1d, d->grav, d->dry_tolerance, amdq, bmasdq, bpasdq);
     else
         rpt2_euler(ixy, n, g, meqn, q1d, d->gamma, amdq, bmasdq, bpasdq);
     for (int i = 1; i <= n + 1; ++i)
@@ -472,6 +593,8 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
         }
     if (d->solver == ORC_SWE)
         rpt2_swe(ixy, n, g, meqn, q1d, d->grav, apdq, bmasdq, bpasdq);
+    else if (d->solver == ORC_SWE_BATHY)
+        rpt2_bathy(ixy, n, g, meqn, q1d, d->grav, d->dry_tolerance, apdq, bmasdq, bpasdq);
     else
         rpt2_euler(ixy, n, g, meqn, q1d, d->gamma, apdq, bmasdq, bpasdq);
     for (int i = 1; i <= n + 1; ++i)
@@ -483,6 +606,8 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
 
 /* qbc element, 1-based cell indices with ghosts */
 #define QB(arr, m, i, j) arr[(m) + meqn * (((i) + g - 1) + (size_t)(mx + 2 * g) * ((j) + g - 1))]
+/* auxbc element (maux components), same indexing */
+#define AB(arr, m, i, j) arr[(m) + maux * (((i) + g - 1) + (size_t)(mx + 2 * g) * ((j) + g - 1))]
 
 /* ------------------------------------------------------------------------------------------
  * step2 (unsplit).  Follows pyclaw/src/pyclaw/classic/step2.f90 (upstream, absent): x-sweeps
@@ -492,13 +617,16 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
  * Returns cfl = max over both sweeps.
  * ---------------------------------------------------------------------------------------- */
 /* One x-sweep slice (row j) of step2, accumulated into qnew. */
-static void xsweep_row(const orc_desc *d, int j, const double *qold, double *qnew, orc_work *wk,
-                       double dtdx, double dtdy, double *cfl) {
-    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx;
+static void xsweep_row(const orc_desc *d, int j, const double *qold, const double *auxbc,
+                       double *qnew, orc_work *wk, double dtdx, double dtdy, double *cfl) {
+    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, maux = d->num_aux;
     orc_work w = *wk;
     double cfl1d;
-    for (int i = 1 - g; i <= mx + g; ++i)
+    for (int i = 1 - g; i <= mx + g; ++i) {
         for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (i + g - 1)] = QB(qold, m, i, j);
+        if (auxbc)
+            for (int m = 0; m < maux; ++m) w.aux1d[m + maux * (i + g - 1)] = AB(auxbc, m, i, j);
+    }
     flux2(d, 1, mx, dtdx, &w, &cfl1d);
     *cfl = fmax(*cfl, cfl1d);
     const double *qadd = w.qadd, *fadd = w.fadd, *gadd = w.gadd;
@@ -513,13 +641,16 @@ static void xsweep_row(const orc_desc *d, int j, const double *qold, double *qne
 }
 
 /* One y-sweep slice (column i) of step2, accumulated into qnew. */
-static void ysweep_col(const orc_desc *d, int i, const double *qold, double *qnew, orc_work *wk,
-                       double dtdx, double dtdy, double *cfl) {
-    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
+static void ysweep_col(const orc_desc *d, int i, const double *qold, const double *auxbc,
+                       double *qnew, orc_work *wk, double dtdx, double dtdy, double *cfl) {
+    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my, maux = d->num_aux;
     orc_work w = *wk;
     double cfl1d;
-    for (int j = 1 - g; j <= my + g; ++j)
+    for (int j = 1 - g; j <= my + g; ++j) {
         for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (j + g - 1)] = QB(qold, m, i, j);
+        if (auxbc)
+            for (int m = 0; m < maux; ++m) w.aux1d[m + maux * (j + g - 1)] = AB(auxbc, m, i, j);
+    }
     flux2(d, 2, my, dtdy, &w, &cfl1d);
     *cfl = fmax(*cfl, cfl1d);
     const double *qadd = w.qadd, *fadd = w.fadd, *gadd = w.gadd;
@@ -540,7 +671,10 @@ static void ysweep_col(const orc_desc *d, int i, const double *qold, double *qne
 static int g_threads = 1;
 void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 
-int orc_step2(const orc_desc *d, const double *qold, double *qnew, double dt, double *cfl_out) {
+/* step2 with aux arrays (auxbc: ghost-filled aux(num_aux, 1-g:mx+g, 1-g:my+g), or NULL when
+ * num_aux = 0). */
+int orc_step2_aux(const orc_desc *d, const double *qold, const double *auxbc, double *qnew,
+                  double dt, double *cfl_out) {
     const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
     const int maxm = mx > my ? mx : my;
     const size_t total = (size_t)meqn * (mx + 2 * g) * (my + 2 * g);
@@ -550,9 +684,9 @@ int orc_step2(const orc_desc *d, const double *qold, double *qnew, double dt, do
     int err = 0;
     if (g_threads <= 1) {
         orc_work w;
-        if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves) != 0) { work_free(&w); return -1; }
-        for (int j = 0; j <= my + 1; ++j) xsweep_row(d, j, qold, qnew, &w, dtdx, dtdy, &cfl);
-        for (int i = 0; i <= mx + 1; ++i) ysweep_col(d, i, qold, qnew, &w, dtdx, dtdy, &cfl);
+        if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves, d->num_aux) != 0) { work_free(&w); return -1; }
+        for (int j = 0; j <= my + 1; ++j) xsweep_row(d, j, qold, auxbc, qnew, &w, dtdx, dtdy, &cfl);
+        for (int i = 0; i <= mx + 1; ++i) ysweep_col(d, i, qold, auxbc, qnew, &w, dtdx, dtdy, &cfl);
         work_free(&w);
     } else {
         for (int dir = 0; dir < 2; ++dir)
@@ -561,12 +695,12 @@ int orc_step2(const orc_desc *d, const double *qold, double *qnew, double dt, do
 #pragma omp parallel num_threads(g_threads) reduction(max : cfl) reduction(| : err)
                 {
                     orc_work w;
-                    if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves) != 0) err = 1;
+                    if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves, d->num_aux) != 0) err = 1;
                     else {
 #pragma omp for schedule(static)
                         for (int k = colour; k <= last; k += 3) {
-                            if (dir == 0) xsweep_row(d, k, qold, qnew, &w, dtdx, dtdy, &cfl);
-                            else ysweep_col(d, k, qold, qnew, &w, dtdx, dtdy, &cfl);
+                            if (dir == 0) xsweep_row(d, k, qold, auxbc, qnew, &w, dtdx, dtdy, &cfl);
+                            else ysweep_col(d, k, qold, auxbc, qnew, &w, dtdx, dtdy, &cfl);
                         }
                     }
                     work_free(&w);
@@ -577,6 +711,10 @@ int orc_step2(const orc_desc *d, const double *qold, double *qnew, double dt, do
     return err ? -1 : 0;
 }
 
+int orc_step2(const orc_desc *d, const double *qold, double *qnew, double dt, double *cfl_out) {
+    return orc_step2_aux(d, qold, NULL, qnew, dt, cfl_out);
+}
+
 /* ------------------------------------------------------------------------------------------
  * Ghost fill.  Follows pyclaw/src/pyclaw/solver.py _apply_bcs/_bc_lower/_bc_upper (upstream,
  * absent): interior copied in, then dimension x over all rows (ghost rows included), then
@@ -584,8 +722,11 @@ int orc_step2(const orc_desc *d, const double *qold, double *qnew, double dt, do
  * x-filled ghosts.  extrap: copy the nearest interior cell; periodic: wrap; wall: mirror with
  * the normal momentum (reflect_index) negated.
  * ---------------------------------------------------------------------------------------- */
-void orc_fill_ghosts(const orc_desc *d, const orc_ctrl *c, const double *q, double *qbc) {
-    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
+/* Ghost fill of an array with `meqn` components under the given BC codes; rf[k] >= 0 is the
+ * component a wall negates in dimension k (q), rf[k] < 0 a plain mirror (aux arrays). */
+static void fill_ghosts_impl(const orc_desc *d, int meqn, const int *bclo, const int *bchi,
+                             const int *rf, const double *q, double *qbc) {
+    const int g = d->num_ghost, mx = d->mx, my = d->my;
     for (int j = 1; j <= my; ++j)
         for (int i = 1; i <= mx; ++i)
             for (int m = 0; m < meqn; ++m)
@@ -595,25 +736,25 @@ void orc_fill_ghosts(const orc_desc *d, const orc_ctrl *c, const double *q, doub
         for (int k = 0; k < g; ++k) {
             const int il = 1 - g + k, iu = mx + g - k; /* lower ghost, upper ghost */
             for (int m = 0; m < meqn; ++m) {
-                switch (c->bc_lower[0]) {
+                switch (bclo[0]) {
                 case BC_EXTRAP: QB(qbc, m, il, j) = QB(qbc, m, 1, j); break;
                 case BC_PERIODIC: QB(qbc, m, il, j) = QB(qbc, m, il + mx, j); break;
                 case BC_WALL: QB(qbc, m, il, j) = QB(qbc, m, 1 - il, j); break;
                 default: break;
                 }
-                switch (c->bc_upper[0]) {
+                switch (bchi[0]) {
                 case BC_EXTRAP: QB(qbc, m, iu, j) = QB(qbc, m, mx, j); break;
                 case BC_PERIODIC: QB(qbc, m, iu, j) = QB(qbc, m, iu - mx, j); break;
                 case BC_WALL: QB(qbc, m, iu, j) = QB(qbc, m, 2 * mx + 1 - iu, j); break;
                 default: break;
                 }
             }
-            if (c->bc_lower[0] == BC_WALL) {
-                const int m = c->reflect_index[0];
+            if (bclo[0] == BC_WALL && rf[0] >= 0) {
+                const int m = rf[0];
                 QB(qbc, m, il, j) = -QB(qbc, m, 1 - il, j);
             }
-            if (c->bc_upper[0] == BC_WALL) {
-                const int m = c->reflect_index[0];
+            if (bchi[0] == BC_WALL && rf[0] >= 0) {
+                const int m = rf[0];
                 QB(qbc, m, iu, j) = -QB(qbc, m, 2 * mx + 1 - iu, j);
             }
         }
@@ -623,29 +764,40 @@ void orc_fill_ghosts(const orc_desc *d, const orc_ctrl *c, const double *q, doub
         for (int k = 0; k < g; ++k) {
             const int jl = 1 - g + k, ju = my + g - k;
             for (int m = 0; m < meqn; ++m) {
-                switch (c->bc_lower[1]) {
+                switch (bclo[1]) {
                 case BC_EXTRAP: QB(qbc, m, i, jl) = QB(qbc, m, i, 1); break;
                 case BC_PERIODIC: QB(qbc, m, i, jl) = QB(qbc, m, i, jl + my); break;
                 case BC_WALL: QB(qbc, m, i, jl) = QB(qbc, m, i, 1 - jl); break;
                 default: break;
                 }
-                switch (c->bc_upper[1]) {
+                switch (bchi[1]) {
                 case BC_EXTRAP: QB(qbc, m, i, ju) = QB(qbc, m, i, my); break;
                 case BC_PERIODIC: QB(qbc, m, i, ju) = QB(qbc, m, i, ju - my); break;
                 case BC_WALL: QB(qbc, m, i, ju) = QB(qbc, m, i, 2 * my + 1 - ju); break;
                 default: break;
                 }
             }
-            if (c->bc_lower[1] == BC_WALL) {
-                const int m = c->reflect_index[1];
+            if (bclo[1] == BC_WALL && rf[1] >= 0) {
+                const int m = rf[1];
                 QB(qbc, m, i, jl) = -QB(qbc, m, i, 1 - jl);
             }
-            if (c->bc_upper[1] == BC_WALL) {
-                const int m = c->reflect_index[1];
+            if (bchi[1] == BC_WALL && rf[1] >= 0) {
+                const int m = rf[1];
                 QB(qbc, m, i, ju) = -QB(qbc, m, i, 2 * my + 1 - ju);
             }
         }
     }
+}
+
+void orc_fill_ghosts(const orc_desc *d, const orc_ctrl *c, const double *q, double *qbc) {
+    fill_ghosts_impl(d, d->num_eqn, c->bc_lower, c->bc_upper, c->reflect_index, q, qbc);
+}
+
+/* aux ghosts (pyclaw _apply_bcs on aux with aux_bc_lower/upper: a wall mirrors every
+ * component without negating any) */
+void orc_fill_aux_ghosts(const orc_desc *d, const orc_ctrl *c, const double *aux, double *auxbc) {
+    const int none[2] = {-1, -1};
+    fill_ghosts_impl(d, d->num_aux, c->aux_bc_lower, c->aux_bc_upper, none, aux, auxbc);
 }
 
 /* One hyperbolic step on interior q (num_eqn, mx, my Fortran order), in place.
@@ -657,22 +809,25 @@ void orc_fill_ghosts(const orc_desc *d, const orc_ctrl *c, const double *q, doub
  * direction ids (1: rows, x-sweeps; 2: columns, y-sweeps) with transverse propagation off
  * (method(3) < 0), qnew += qadd - dt/dx (fadd(i+1) - fadd(i)).  Returns the sweep's CFL.
  * ---------------------------------------------------------------------------------------- */
-int orc_step2ds(const orc_desc *d, const double *qold, double *qnew, double dt, int ids,
-                double *cfl_out) {
-    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
+int orc_step2ds_aux(const orc_desc *d, const double *qold, const double *auxbc, double *qnew,
+                    double dt, int ids, double *cfl_out) {
+    const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my, maux = d->num_aux;
     const int maxm = mx > my ? mx : my;
     orc_desc dd = *d;
     dd.transverse_waves = 0;
     orc_work w;
-    if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves) != 0) { work_free(&w); return -1; }
+    if (work_alloc(&w, maxm + 2 * g, meqn, d->num_waves, maux) != 0) { work_free(&w); return -1; }
     const size_t total = (size_t)meqn * (mx + 2 * g) * (my + 2 * g);
     memcpy(qnew, qold, total * sizeof(double));
     double cfl = 0.0, cfl1d;
     if (ids == 1) {
         const double dtdx = dt / d->dx;
         for (int j = 1; j <= my; ++j) {
-            for (int i = 1 - g; i <= mx + g; ++i)
+            for (int i = 1 - g; i <= mx + g; ++i) {
                 for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (i + g - 1)] = QB(qold, m, i, j);
+                if (auxbc)
+                    for (int m = 0; m < maux; ++m) w.aux1d[m + maux * (i + g - 1)] = AB(auxbc, m, i, j);
+            }
             flux2(&dd, 1, mx, dtdx, &w, &cfl1d);
             cfl = fmax(cfl, cfl1d);
             const double *qadd = w.qadd, *fadd = w.fadd;
@@ -684,8 +839,11 @@ int orc_step2ds(const orc_desc *d, const double *qold, double *qnew, double dt, 
     } else {
         const double dtdy = dt / d->dy;
         for (int i = 1; i <= mx; ++i) {
-            for (int j = 1 - g; j <= my + g; ++j)
+            for (int j = 1 - g; j <= my + g; ++j) {
                 for (int m = 0; m < meqn; ++m) w.q1d[m + meqn * (j + g - 1)] = QB(qold, m, i, j);
+                if (auxbc)
+                    for (int m = 0; m < maux; ++m) w.aux1d[m + maux * (j + g - 1)] = AB(auxbc, m, i, j);
+            }
             flux2(&dd, 2, my, dtdy, &w, &cfl1d);
             cfl = fmax(cfl, cfl1d);
             const double *qadd = w.qadd, *fadd = w.fadd;
@@ -700,6 +858,11 @@ int orc_step2ds(const orc_desc *d, const double *qold, double *qnew, double dt, 
     return 0;
 }
 
+int orc_step2ds(const orc_desc *d, const double *qold, double *qnew, double dt, int ids,
+                double *cfl_out) {
+    return orc_step2ds_aux(d, qold, NULL, qnew, dt, ids, cfl_out);
+}
+
 static void copy_interior(const orc_desc *d, const double *qbc, double *q) {
     const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
     for (int j = 1; j <= my; ++j)
@@ -711,30 +874,39 @@ static void copy_interior(const orc_desc *d, const double *qbc, double *q) {
 /* ClawSolver2D.step_hyperbolic (classic/solver.py, upstream): ghost fill, then step2, or with
  * dimensional splitting the x-sweeps of step2ds, a fresh ghost fill of the intermediate state and
  * the y-sweeps (Godunov splitting, x first); cfl = max of the two sweeps. */
-int orc_step_hyperbolic(const orc_desc *d, const orc_ctrl *c, double *q, double dt, double *cfl) {
+int orc_step_hyperbolic_aux(const orc_desc *d, const orc_ctrl *c, double *q, const double *aux,
+                            double dt, double *cfl) {
     const int g = d->num_ghost, meqn = d->num_eqn, mx = d->mx, my = d->my;
     const size_t nbc = (size_t)meqn * (mx + 2 * g) * (my + 2 * g);
+    const size_t nabc = (size_t)(d->num_aux > 0 ? d->num_aux : 0) * (mx + 2 * g) * (my + 2 * g);
     double *qold = malloc(nbc * sizeof(double));
     double *qnew = malloc(nbc * sizeof(double));
-    if (!qold || !qnew) { free(qold); free(qnew); return -1; }
+    double *auxbc = (aux && nabc) ? malloc(nabc * sizeof(double)) : NULL;
+    if (!qold || !qnew || (aux && nabc && !auxbc)) { free(qold); free(qnew); free(auxbc); return -1; }
+    if (auxbc) orc_fill_aux_ghosts(d, c, aux, auxbc);
     orc_fill_ghosts(d, c, q, qold);
     int rc;
     if (c->dimensional_split) {
         double cx = 0.0, cy = 0.0;
-        rc = orc_step2ds(d, qold, qnew, dt, 1, &cx);
+        rc = orc_step2ds_aux(d, qold, auxbc, qnew, dt, 1, &cx);
         if (rc == 0) {
             copy_interior(d, qnew, q);
             orc_fill_ghosts(d, c, q, qold);
-            rc = orc_step2ds(d, qold, qnew, dt, 2, &cy);
+            rc = orc_step2ds_aux(d, qold, auxbc, qnew, dt, 2, &cy);
         }
         *cfl = fmax(cx, cy);
     } else {
-        rc = orc_step2(d, qold, qnew, dt, cfl);
+        rc = orc_step2_aux(d, qold, auxbc, qnew, dt, cfl);
     }
     if (rc == 0) copy_interior(d, qnew, q);
     free(qold);
     free(qnew);
+    free(auxbc);
     return rc;
+}
+
+int orc_step_hyperbolic(const orc_desc *d, const orc_ctrl *c, double *q, double dt, double *cfl) {
+    return orc_step_hyperbolic_aux(d, c, q, NULL, dt, cfl);
 }
 
 /* ------------------------------------------------------------------------------------------
@@ -745,8 +917,8 @@ int orc_step_hyperbolic(const orc_desc *d, const orc_ctrl *c, double *q, double 
  * unchanged); nsteps_target <= 0 runs to tend.  st->dt carries dt in and out (set it to
  * dt_initial before the first call).
  * ---------------------------------------------------------------------------------------- */
-int orc_evolve(const orc_desc *d, const orc_ctrl *c, double *q, double tend, long nsteps_target,
-               orc_status *st) {
+int orc_evolve_aux(const orc_desc *d, const orc_ctrl *c, double *q, const double *aux, double tend,
+                   long nsteps_target, orc_status *st) {
     const size_t n = (size_t)d->num_eqn * d->mx * d->my;
     double *qbackup = malloc(n * sizeof(double));
     if (!qbackup) return -1;
@@ -764,7 +936,7 @@ int orc_evolve(const orc_desc *d, const orc_ctrl *c, double *q, double tend, lon
         const double told = st->t;
         if (c->dt_variable) memcpy(qbackup, q, n * sizeof(double));
         double cfl = 0.0;
-        if (orc_step_hyperbolic(d, c, q, st->dt, &cfl) != 0) { free(qbackup); return -1; }
+        if (orc_step_hyperbolic_aux(d, c, q, aux, st->dt, &cfl) != 0) { free(qbackup); return -1; }
         st->attempts += 1;
         st->last_cfl = cfl;
         if (!(cfl > c->cfl_max)) {
@@ -808,17 +980,31 @@ int orc_evolve(const orc_desc *d, const orc_ctrl *c, double *q, double tend, lon
     return 0;
 }
 
+int orc_evolve(const orc_desc *d, const orc_ctrl *c, double *q, double tend, long nsteps_target,
+               orc_status *st) {
+    return orc_evolve_aux(d, c, q, NULL, tend, nsteps_target, st);
+}
+
 /* ------------------------------------------------------------------------------------------
  * Unit entry points for tests: one slice of rpn2 / rpt2 / flux2 on a q1d(meqn, 1-g:n+g).
  * ---------------------------------------------------------------------------------------- */
-int orc_rpn2(const orc_desc *d, int ixy, int n, const double *q1d_in, double *wave_out,
-             double *s_out, double *amdq_out, double *apdq_out) {
+int orc_rpn2_aux(const orc_desc *d, int ixy, int n, const double *q1d_in, const double *aux1d_in,
+                 double *wave_out, double *s_out, double *amdq_out, double *apdq_out) {
     const int g = d->num_ghost, meqn = d->num_eqn, mwaves = d->num_waves;
     if (d->solver == ORC_SWE)
         rpn2_swe(ixy, n, g, meqn, mwaves, q1d_in, d->grav, wave_out, s_out, amdq_out, apdq_out);
+    else if (d->solver == ORC_SWE_BATHY)
+        rpn2_bathy(ixy, n, g, meqn, mwaves, q1d_in, aux1d_in, d->num_aux, d->grav, d->dry_tolerance,
+                   wave_out, s_out, amdq_out, apdq_out);
     else
         rpn2_euler(ixy, n, g, meqn, mwaves, q1d_in, d->gamma, wave_out, s_out, amdq_out, apdq_out);
     return 0;
+}
+
+int orc_rpn2(const orc_desc *d, int ixy, int n, const double *q1d_in, double *wave_out,
+             double *s_out, double *amdq_out, double *apdq_out) {
+    if (d->num_aux > 0) return -1;
+    return orc_rpn2_aux(d, ixy, n, q1d_in, NULL, wave_out, s_out, amdq_out, apdq_out);
 }
 
 int orc_rpt2(const orc_desc *d, int ixy, int n, const double *q1d_in, const double *asdq,
@@ -826,6 +1012,8 @@ int orc_rpt2(const orc_desc *d, int ixy, int n, const double *q1d_in, const doub
     const int g = d->num_ghost, meqn = d->num_eqn;
     if (d->solver == ORC_SWE)
         rpt2_swe(ixy, n, g, meqn, q1d_in, d->grav, asdq, bmasdq, bpasdq);
+    else if (d->solver == ORC_SWE_BATHY)
+        rpt2_bathy(ixy, n, g, meqn, q1d_in, d->grav, d->dry_tolerance, asdq, bmasdq, bpasdq);
     else
         rpt2_euler(ixy, n, g, meqn, q1d_in, d->gamma, asdq, bmasdq, bpasdq);
     return 0;
@@ -837,19 +1025,28 @@ int orc_limiter(const orc_desc *d, int n, double *wave, const double *s) {
     return 0;
 }
 
-/* flux2 on one slice: outputs qadd, fadd (meqn x len) and gadd (meqn x 2 x len), cfl1d */
-int orc_flux2(const orc_desc *d, int ixy, int n, double dtdx, const double *q1d_in,
-              double *qadd_out, double *fadd_out, double *gadd_out, double *cfl1d) {
+/* flux2 on one slice: outputs qadd, fadd (meqn x len) and gadd (meqn x 2 x len), cfl1d;
+ * aux1d_in: aux(num_aux, len) of the slice, or NULL when num_aux = 0 */
+int orc_flux2_aux(const orc_desc *d, int ixy, int n, double dtdx, const double *q1d_in,
+                  const double *aux1d_in, double *qadd_out, double *fadd_out, double *gadd_out,
+                  double *cfl1d) {
     const int g = d->num_ghost, meqn = d->num_eqn;
+    if (d->num_aux > 0 && !aux1d_in) return -1;
     orc_work w;
-    if (work_alloc(&w, n + 2 * g, meqn, d->num_waves) != 0) { work_free(&w); return -1; }
+    if (work_alloc(&w, n + 2 * g, meqn, d->num_waves, d->num_aux) != 0) { work_free(&w); return -1; }
     memcpy(w.q1d, q1d_in, sizeof(double) * meqn * (n + 2 * g));
+    if (d->num_aux > 0) memcpy(w.aux1d, aux1d_in, sizeof(double) * d->num_aux * (n + 2 * g));
     flux2(d, ixy, n, dtdx, &w, cfl1d);
     memcpy(qadd_out, w.qadd, sizeof(double) * meqn * (n + 2 * g));
     memcpy(fadd_out, w.fadd, sizeof(double) * meqn * (n + 2 * g));
     memcpy(gadd_out, w.gadd, sizeof(double) * meqn * 2 * (n + 2 * g));
     work_free(&w);
     return 0;
+}
+
+int orc_flux2(const orc_desc *d, int ixy, int n, double dtdx, const double *q1d_in,
+              double *qadd_out, double *fadd_out, double *gadd_out, double *cfl1d) {
+    return orc_flux2_aux(d, ixy, n, dtdx, q1d_in, NULL, qadd_out, fadd_out, gadd_out, cfl1d);
 }
 
 /* Compensated (Neumaier) sum of one component over the interior, for conservation checks. */

@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liborc2d.so")
 
-SWE, EULER = 0, 1
+SWE, EULER, SWE_BATHY = 0, 1, 2
 BC_CUSTOM, BC_EXTRAP, BC_PERIODIC, BC_WALL = 0, 1, 2, 3
 
 
@@ -22,6 +22,7 @@ class Desc(C.Structure):
         ("num_ghost", C.c_int), ("mx", C.c_int), ("my", C.c_int), ("order", C.c_int),
         ("transverse_waves", C.c_int), ("mthlim", C.c_int * 4), ("dx", C.c_double),
         ("dy", C.c_double), ("grav", C.c_double), ("gamma", C.c_double),
+        ("num_aux", C.c_int), ("fwave", C.c_int), ("dry_tolerance", C.c_double),
     ]
 
 
@@ -30,7 +31,7 @@ class Ctrl(C.Structure):
         ("dt_initial", C.c_double), ("dt_max", C.c_double), ("cfl_max", C.c_double),
         ("cfl_desired", C.c_double), ("dt_variable", C.c_int), ("max_steps", C.c_int),
         ("bc_lower", C.c_int * 2), ("bc_upper", C.c_int * 2), ("reflect_index", C.c_int * 2),
-        ("dimensional_split", C.c_int),
+        ("dimensional_split", C.c_int), ("aux_bc_lower", C.c_int * 2), ("aux_bc_upper", C.c_int * 2),
     ]
 
 
@@ -59,11 +60,23 @@ def lib():
         dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
         dpa = C.POINTER(C.c_double)
         L.orc_step2.argtypes = [C.POINTER(Desc), dpa, dpa, C.c_double, dpa]
+        L.orc_step2_aux.argtypes = [C.POINTER(Desc), dpa, dpa, dpa, C.c_double, dpa]
+        L.orc_step2ds_aux.argtypes = [C.POINTER(Desc), dpa, dpa, dpa, C.c_double, C.c_int, dpa]
         L.orc_fill_ghosts.argtypes = [C.POINTER(Desc), C.POINTER(Ctrl), dpa, dpa]
+        L.orc_fill_ghosts.restype = None
+        L.orc_fill_aux_ghosts.argtypes = [C.POINTER(Desc), C.POINTER(Ctrl), dpa, dpa]
+        L.orc_fill_aux_ghosts.restype = None
         L.orc_step_hyperbolic.argtypes = [C.POINTER(Desc), C.POINTER(Ctrl), dpa, C.c_double, dpa]
+        L.orc_step_hyperbolic_aux.argtypes = [C.POINTER(Desc), C.POINTER(Ctrl), dpa, dpa, C.c_double,
+                                              dpa]
         L.orc_evolve.argtypes = [C.POINTER(Desc), C.POINTER(Ctrl), dpa, C.c_double, C.c_long,
                                  C.POINTER(Status)]
+        L.orc_evolve_aux.argtypes = [C.POINTER(Desc), C.POINTER(Ctrl), dpa, dpa, C.c_double,
+                                     C.c_long, C.POINTER(Status)]
         L.orc_rpn2.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, dpa, dpa, dpa, dpa, dpa]
+        L.orc_rpn2_aux.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, dpa, dpa, dpa, dpa, dpa, dpa]
+        L.orc_flux2_aux.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, C.c_double, dpa, dpa, dpa,
+                                    dpa, dpa, dpa]
         L.orc_rpt2.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, dpa, dpa, dpa, dpa]
         L.orc_limiter.argtypes = [C.POINTER(Desc), C.c_int, dpa, dpa]
         L.orc_flux2.argtypes = [C.POINTER(Desc), C.c_int, C.c_int, C.c_double, dpa, dpa, dpa,
@@ -74,8 +87,9 @@ def lib():
         L.orc_set_threads.restype = None
         L.orc_component_sum.argtypes = [C.POINTER(Desc), dpa, C.c_int]
         L.orc_component_sum.restype = C.c_double
-        for f in ("orc_step2", "orc_step_hyperbolic", "orc_evolve", "orc_rpn2", "orc_rpt2",
-                  "orc_limiter", "orc_flux2"):
+        for f in ("orc_step2", "orc_step2_aux", "orc_step2ds_aux", "orc_step_hyperbolic",
+                  "orc_step_hyperbolic_aux", "orc_evolve", "orc_evolve_aux", "orc_rpn2",
+                  "orc_rpn2_aux", "orc_rpt2", "orc_limiter", "orc_flux2", "orc_flux2_aux"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -88,15 +102,30 @@ def set_threads(n):
 
 
 def _p(a):
-    return a.ctypes.data_as(C.POINTER(C.c_double))
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _solver_id(solver):
+    if solver in (SWE, "swe", "shallow_roe_with_efix_2D"):
+        return SWE
+    if solver in (SWE_BATHY, "bathy", "shallow_bathymetry_fwave_2D"):
+        return SWE_BATHY
+    if solver in (EULER, "euler", "euler_4wave_2D"):
+        return EULER
+    raise ValueError("unknown solver %r" % (solver,))
 
 
 def make_desc(solver, mx, my, dx, dy, grav=1.0, gamma=1.4, order=2, transverse_waves=2,
-              mthlim=(4, 4, 4, 4), num_ghost=2):
+              mthlim=(4, 4, 4, 4), num_ghost=2, num_aux=None, dry_tolerance=1e-3, fwave=None):
+    """fwave and num_aux default to what the solver needs (f-waves and one aux field, the
+    bathymetry, for shallow_bathymetry_fwave_2D)."""
     d = Desc()
-    d.solver = SWE if solver in (SWE, "swe", "shallow_roe_with_efix_2D") else EULER
-    d.num_eqn = 3 if d.solver == SWE else 4
-    d.num_waves = 3 if d.solver == SWE else 4
+    d.solver = _solver_id(solver)
+    d.num_eqn = 4 if d.solver == EULER else 3
+    d.num_waves = d.num_eqn
+    d.num_aux = int((1 if d.solver == SWE_BATHY else 0) if num_aux is None else num_aux)
+    d.fwave = int(bool(d.solver == SWE_BATHY if fwave is None else fwave))
+    d.dry_tolerance = float(dry_tolerance)
     d.num_ghost = num_ghost
     d.mx, d.my, d.order, d.transverse_waves = mx, my, order, transverse_waves
     lim = list(mthlim) + [mthlim[-1]] * (4 - len(mthlim))
@@ -108,14 +137,25 @@ def make_desc(solver, mx, my, dx, dy, grav=1.0, gamma=1.4, order=2, transverse_w
 
 def make_ctrl(bc_lower=(1, 1), bc_upper=(1, 1), reflect_index=(1, 2), dt_initial=0.1,
               dt_max=1e99, cfl_max=1.0, cfl_desired=0.9, dt_variable=True, max_steps=10000,
-              dimensional_split=False):
+              dimensional_split=False, aux_bc_lower=None, aux_bc_upper=None):
+    """aux_bc_lower/upper default to bc_lower/bc_upper."""
     c = Ctrl()
     c.dt_initial, c.dt_max, c.cfl_max, c.cfl_desired = dt_initial, dt_max, cfl_max, cfl_desired
     c.dt_variable, c.max_steps = int(bool(dt_variable)), int(max_steps)
     c.dimensional_split = int(bool(dimensional_split))
     for k in range(2):
         c.bc_lower[k], c.bc_upper[k], c.reflect_index[k] = bc_lower[k], bc_upper[k], reflect_index[k]
+        c.aux_bc_lower[k] = (bc_lower if aux_bc_lower is None else aux_bc_lower)[k]
+        c.aux_bc_upper[k] = (bc_upper if aux_bc_upper is None else aux_bc_upper)[k]
     return c
+
+
+def _aux_arg(d, aux):
+    if d.num_aux == 0:
+        return None
+    if aux is None:
+        raise ValueError("this solver needs aux (num_aux = %d)" % d.num_aux)
+    return np.asfortranarray(aux, dtype=np.float64)
 
 
 def fill_ghosts(d, c, q):
@@ -126,34 +166,56 @@ def fill_ghosts(d, c, q):
     return qbc
 
 
-def step2(d, qbc, dt):
-    """One f2py-style step2 call on a ghost-filled qbc -> (qnew_bc, cfl)."""
+def fill_aux_ghosts(d, c, aux):
+    aux = np.asfortranarray(aux, dtype=np.float64)
+    g = d.num_ghost
+    auxbc = np.zeros((d.num_aux, d.mx + 2 * g, d.my + 2 * g), order="F")
+    lib().orc_fill_aux_ghosts(C.byref(d), C.byref(c), _p(aux), _p(auxbc))
+    return auxbc
+
+
+def step2(d, qbc, dt, auxbc=None):
+    """One f2py-style step2 call on a ghost-filled qbc (and auxbc) -> (qnew_bc, cfl)."""
     qold = np.asfortranarray(qbc, dtype=np.float64)
     qnew = np.zeros_like(qold, order="F")
     cfl = C.c_double(0.0)
-    rc = lib().orc_step2(C.byref(d), _p(qold), _p(qnew), dt, C.byref(cfl))
+    rc = lib().orc_step2_aux(C.byref(d), _p(qold), _p(_aux_arg(d, auxbc)), _p(qnew), dt,
+                             C.byref(cfl))
     assert rc == 0
     return qnew, cfl.value
 
 
-def step_hyperbolic(d, c, q, dt):
+def step2ds(d, qbc, dt, ids, auxbc=None):
+    """One f2py-style step2ds call (ids 1: x-sweeps, 2: y-sweeps) -> (qnew_bc, cfl)."""
+    qold = np.asfortranarray(qbc, dtype=np.float64)
+    qnew = np.zeros_like(qold, order="F")
+    cfl = C.c_double(0.0)
+    rc = lib().orc_step2ds_aux(C.byref(d), _p(qold), _p(_aux_arg(d, auxbc)), _p(qnew), dt,
+                               int(ids), C.byref(cfl))
+    assert rc == 0
+    return qnew, cfl.value
+
+
+def step_hyperbolic(d, c, q, dt, aux=None):
     """ClawSolver2D.step_hyperbolic on interior q -> (qnew, cfl); q is not modified."""
     qq = np.array(q, dtype=np.float64, order="F", copy=True)
     cfl = C.c_double(0.0)
-    rc = lib().orc_step_hyperbolic(C.byref(d), C.byref(c), _p(qq), dt, C.byref(cfl))
+    rc = lib().orc_step_hyperbolic_aux(C.byref(d), C.byref(c), _p(qq), _p(_aux_arg(d, aux)), dt,
+                                       C.byref(cfl))
     assert rc == 0
     return qq, cfl.value
 
 
-def evolve(d, c, q, nsteps=None, tend=None, t0=0.0, dt=None):
+def evolve(d, c, q, nsteps=None, tend=None, t0=0.0, dt=None, aux=None):
     """Solver.evolve_to_time semantics.  nsteps: stop after that many accepted steps."""
     qq = np.array(q, dtype=np.float64, order="F", copy=True)
     st = Status()
     st.t = t0
     st.dt = c.dt_initial if dt is None else dt
     st.cflmax = 0.0
-    rc = lib().orc_evolve(C.byref(d), C.byref(c), _p(qq), float("inf") if tend is None else tend,
-                          int(nsteps or 0), C.byref(st))
+    rc = lib().orc_evolve_aux(C.byref(d), C.byref(c), _p(qq), _p(_aux_arg(d, aux)),
+                              float("inf") if tend is None else tend, int(nsteps or 0),
+                              C.byref(st))
     assert rc == 0
     status = {k: getattr(st, k) for k, _ in Status._fields_}
     return qq, status
@@ -165,16 +227,18 @@ def slice_arrays(d, n):
     return L, g
 
 
-def rpn2(d, ixy, q1d):
-    """q1d: (num_eqn, n+2g) -> wave (meqn, mwaves, L), s (mwaves, L), amdq, apdq (meqn, L)."""
+def rpn2(d, ixy, q1d, aux1d=None):
+    """q1d: (num_eqn, n+2g) [aux1d: (num_aux, n+2g)] -> wave (meqn, mwaves, L), s (mwaves, L),
+    amdq, apdq (meqn, L)."""
     q1d = np.asfortranarray(q1d, dtype=np.float64)
+    aux1d = _aux_arg(d, aux1d)
     L = q1d.shape[1]
     n = L - 2 * d.num_ghost
     wave = np.zeros((d.num_eqn, d.num_waves, L), order="F")
     s = np.zeros((d.num_waves, L), order="F")
     amdq = np.zeros((d.num_eqn, L), order="F")
     apdq = np.zeros((d.num_eqn, L), order="F")
-    lib().orc_rpn2(C.byref(d), ixy, n, _p(q1d), _p(wave), _p(s), _p(amdq), _p(apdq))
+    lib().orc_rpn2_aux(C.byref(d), ixy, n, _p(q1d), _p(aux1d), _p(wave), _p(s), _p(amdq), _p(apdq))
     return wave, s, amdq, apdq
 
 
@@ -197,15 +261,18 @@ def limiter(d, wave, s):
     return wave
 
 
-def flux2(d, ixy, q1d, dtdx):
+def flux2(d, ixy, q1d, dtdx, aux1d=None):
     q1d = np.asfortranarray(q1d, dtype=np.float64)
+    aux1d = _aux_arg(d, aux1d)
     L = q1d.shape[1]
     n = L - 2 * d.num_ghost
     qadd = np.zeros((d.num_eqn, L), order="F")
     fadd = np.zeros((d.num_eqn, L), order="F")
     gadd = np.zeros((d.num_eqn, 2, L), order="F")
     cfl = C.c_double(0.0)
-    lib().orc_flux2(C.byref(d), ixy, n, dtdx, _p(q1d), _p(qadd), _p(fadd), _p(gadd), C.byref(cfl))
+    rc = lib().orc_flux2_aux(C.byref(d), ixy, n, dtdx, _p(q1d), _p(aux1d), _p(qadd), _p(fadd),
+                             _p(gadd), C.byref(cfl))
+    assert rc == 0
     return qadd, fadd, gadd, cfl.value
 
 

@@ -135,6 +135,60 @@ def rpt2_swe(ql, qr, asdq, grav):
     return bm, bp
 
 
+def _roe_bathy(ql, qr, grav, tol):
+    """Roe averages with zero velocity in dry cells (h <= tol); mask of interfaces with a wet cell."""
+    hl, hr = ql[0], qr[0]
+    wl, wr = hl > tol, hr > tol
+    wet = wl | wr
+    one = torch.ones_like(hl)
+    ul = torch.where(wl, ql[1] / torch.where(wl, hl, one), 0 * hl)
+    vl = torch.where(wl, ql[2] / torch.where(wl, hl, one), 0 * hl)
+    ur = torch.where(wr, qr[1] / torch.where(wr, hr, one), 0 * hr)
+    vr = torch.where(wr, qr[2] / torch.where(wr, hr, one), 0 * hr)
+    sl, sr = torch.sqrt(torch.where(wet, hl, one)), torch.sqrt(torch.where(wet, hr, one))
+    u = (sl * ul + sr * ur) / (sl + sr)
+    v = (sl * vl + sr * vr) / (sl + sr)
+    c = torch.sqrt(grav * (0.5 * torch.where(wet, hl + hr, 2 * one)))
+    return ul, vl, ur, vr, u, v, c, wet
+
+
+def rpn2_bathy(ql, qr, bl, br, grav, tol):
+    """f-wave split of f(q_r) - f(q_l) + (0, g hbar (b_r - b_l), 0) along the Roe eigenvectors;
+    returns f-waves (3 eqn, 3 waves, ...), speeds, amdq, apdq."""
+    hl, hr = ql[0], qr[0]
+    ul, vl, ur, vr, u, v, c, wet = _roe_bathy(ql, qr, grav, tol)
+    d1 = hr * ur - hl * ul
+    d2 = (hr * ur * ur + 0.5 * grav * hr * hr) - (hl * ul * ul + 0.5 * grav * hl * hl) \
+        + grav * (0.5 * (hl + hr)) * (br - bl)
+    d3 = hr * ur * vr - hl * ul * vl
+    s1, s3 = u - c, u + c
+    z = torch.zeros_like(d1)
+    b1 = torch.where(wet, (s3 * d1 - d2) / (s3 - s1), z)
+    b3 = torch.where(wet, (d2 - s1 * d1) / (s3 - s1), z)
+    b2 = torch.where(wet, d3 - v * d1, z)
+    s1, s2, s3 = torch.where(wet, s1, z), torch.where(wet, u, z), torch.where(wet, s3, z)
+    vv = torch.where(wet, v, z)
+    Z = [torch.stack([b1, b1 * s1, b1 * vv]), torch.stack([z, z, b2]), torch.stack([b3, b3 * s3, b3 * vv])]
+    sp = [s1, s2, s3]
+    amdq = sum(torch.where(s < 0, 1.0, torch.where(s > 0, 0.0, 0.5)) * w for s, w in zip(sp, Z))
+    apdq = sum(torch.where(s > 0, 1.0, torch.where(s < 0, 0.0, 0.5)) * w for s, w in zip(sp, Z))
+    return torch.stack(Z, dim=1), torch.stack(sp), amdq, apdq
+
+
+def rpt2_bathy(ql, qr, asdq, grav, tol):
+    _, _, _, _, u, v, a, wet = _roe_bathy(ql, qr, grav, tol)
+    b1 = (0.5 / a) * ((v + a) * asdq[0] - asdq[2])
+    b2 = asdq[1] - u * asdq[0]
+    b3 = (0.5 / a) * (-(v - a) * asdq[0] + asdq[2])
+    z = torch.zeros_like(b1)
+    B = [torch.stack([b1, b1 * u, b1 * (v - a)]), torch.stack([z, b2, z]),
+         torch.stack([b3, b3 * u, b3 * (v + a)])]
+    sb = [v - a, v, v + a]
+    bm = sum(torch.clamp(s, max=0.0) * w for s, w in zip(sb, B))
+    bp = sum(torch.clamp(s, min=0.0) * w for s, w in zip(sb, B))
+    return torch.where(wet, bm, 0 * bm), torch.where(wet, bp, 0 * bp)
+
+
 def _roe_euler(ql, qr, gamma1):
     rl, rr = torch.sqrt(ql[0]), torch.sqrt(qr[0])
     pl = gamma1 * (ql[3] - 0.5 * (ql[1] ** 2 + ql[2] ** 2) / ql[0])
@@ -233,11 +287,15 @@ def philim(theta, meth):
 # Returns per-cell (cells 1..n of each slice, i.e. padded positions g..g+n-1):
 #   qadd, fdiff = fadd(i+1)-fadd(i), g1, g2 (all (meqn, n, M)), and smax over interfaces 1..n+1.
 # ---------------------------------------------------------------------------------------------
-def sweep(Q, r, g, solver, phys, mthlim, order=2, transverse_waves=2):
+def sweep(Q, r, g, solver, phys, mthlim, order=2, transverse_waves=2, B=None):
+    """B: the bathymetry on the same padded slices, (N, M), for solver 'bathy' (phys = (grav,
+    dry_tolerance)); the second-order correction then uses the f-wave form sign(s) for |s|."""
     meqn, N, M = Q.shape
     n = N - 2 * g
     ql, qr = Q[:, :-1], Q[:, 1:]  # interface k between padded cells k and k+1
-    if solver == "swe":
+    if solver == "bathy":
+        waves, s, amdq, apdq = rpn2_bathy(ql, qr, B[:-1], B[1:], phys[0], phys[1])
+    elif solver == "swe":
         waves, s, amdq, apdq = rpn2_swe(ql, qr, phys)
     else:
         waves, s, amdq, apdq = rpn2_euler(ql, qr, phys)
@@ -264,7 +322,8 @@ def sweep(Q, r, g, solver, phys, mthlim, order=2, transverse_waves=2):
         phi = torch.stack(phis)
         wlim = wc * phi.unsqueeze(0)
         sa = sl.abs()
-        cq = (sa * (1.0 - sa * r)).unsqueeze(0) * wlim
+        sgn = torch.where(torch.signbit(sl), -1.0, 1.0) if solver == "bathy" else sa
+        cq = (sgn * (1.0 - sa * r)).unsqueeze(0) * wlim
         cq = cq.sum(1)
         fadd = 0.5 * cq
     am = amdq[:, k0:k1]
@@ -278,7 +337,10 @@ def sweep(Q, r, g, solver, phys, mthlim, order=2, transverse_waves=2):
         am = am + cq
         ap = ap - cq
     qli, qri = ql[:, k0:k1], qr[:, k0:k1]
-    if solver == "swe":
+    if solver == "bathy":
+        bm1, bp1 = rpt2_bathy(qli, qri, am, phys[0], phys[1])
+        bm2, bp2 = rpt2_bathy(qli, qri, ap, phys[0], phys[1])
+    elif solver == "swe":
         bm1, bp1 = rpt2_swe(qli, qri, am, phys)
         bm2, bp2 = rpt2_swe(qli, qri, ap, phys)
     else:
@@ -291,22 +353,33 @@ def sweep(Q, r, g, solver, phys, mthlim, order=2, transverse_waves=2):
 
 
 def _swap_xy(Q, solver):
-    perm = [0, 2, 1] if solver == "swe" else [0, 2, 1, 3]
+    perm = [0, 2, 1] if solver in ("swe", "bathy") else [0, 2, 1, 3]
     return Q[perm].transpose(1, 2)
 
 
-def step2(qbc, dt, dx, dy, g, solver, phys, mthlim, order=2, transverse_waves=2):
-    """qbc (meqn, mx+2g, my+2g) ghost-filled -> (q interior after one step, cfl)."""
+def fill_aux_ghosts(b, g, bc_lower, bc_upper):
+    """b (mx, my) -> (mx+2g, my+2g); a wall mirrors without a sign change."""
+    mx, my = b.shape
+    ix, _ = ghost_index(mx, g, bc_lower[0], bc_upper[0])
+    iy, _ = ghost_index(my, g, bc_lower[1], bc_upper[1])
+    return b[ix][:, iy]
+
+
+def step2(qbc, dt, dx, dy, g, solver, phys, mthlim, order=2, transverse_waves=2, bbc=None):
+    """qbc (meqn, mx+2g, my+2g) ghost-filled -> (q interior after one step, cfl).  bbc: the
+    ghost-filled bathymetry (mx+2g, my+2g) for solver 'bathy'."""
     meqn, NX, NY = qbc.shape
     mx, my = NX - 2 * g, NY - 2 * g
     dtdx, dtdy = dt / dx, dt / dy
     # x-sweeps on rows 0..my+1 (padded g-1 .. g+my)
     Qx = qbc[:, :, g - 1:g + my + 1]
-    qa, fd, g1, g2, smx = sweep(Qx, dtdx, g, solver, phys, mthlim, order, transverse_waves)
+    Bx = None if bbc is None else bbc[:, g - 1:g + my + 1]
+    qa, fd, g1, g2, smx = sweep(Qx, dtdx, g, solver, phys, mthlim, order, transverse_waves, Bx)
     X = qa - dtdx * fd - dtdy * (g2 - g1)  # (meqn, mx, my+2), column c = row c-1
     # y-sweeps on columns 0..mx+1
     Qy = _swap_xy(qbc[:, g - 1:g + mx + 1, :], solver)  # (meqn, NY, mx+2)
-    qa, fd, h1, h2, smy = sweep(Qy, dtdy, g, solver, phys, mthlim, order, transverse_waves)
+    By = None if bbc is None else bbc[g - 1:g + mx + 1, :].transpose(0, 1)
+    qa, fd, h1, h2, smy = sweep(Qy, dtdy, g, solver, phys, mthlim, order, transverse_waves, By)
     Yt = qa - dtdy * fd - dtdx * (h2 - h1)
     Y = _swap_xy(Yt, solver)          # (meqn, mx+2, my)
     H1 = _swap_xy(h1, solver)
@@ -319,23 +392,28 @@ def step2(qbc, dt, dx, dy, g, solver, phys, mthlim, order=2, transverse_waves=2)
 
 
 def step(q, dt, dx, dy, solver, phys, mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_upper=(1, 1),
-         reflect_index=(1, 2), g=2, order=2, transverse_waves=2):
-    """ClawSolver2D.step_hyperbolic on interior q (meqn, mx, my) -> (qnew, cfl)."""
+         reflect_index=(1, 2), g=2, order=2, transverse_waves=2, b=None, aux_bc_lower=None,
+         aux_bc_upper=None):
+    """ClawSolver2D.step_hyperbolic on interior q (meqn, mx, my) -> (qnew, cfl).  b: the
+    bathymetry (mx, my) for solver 'bathy'; its BCs default to the q BCs."""
     qbc = fill_ghosts(q, g, bc_lower, bc_upper, reflect_index)
-    return step2(qbc, dt, dx, dy, g, solver, phys, mthlim, order, transverse_waves)
+    bbc = None if b is None else fill_aux_ghosts(b, g, aux_bc_lower or bc_lower, aux_bc_upper or bc_upper)
+    return step2(qbc, dt, dx, dy, g, solver, phys, mthlim, order, transverse_waves, bbc)
 
 
 def step_split(q, dt, dx, dy, solver, phys, mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_upper=(1, 1),
-               reflect_index=(1, 2), g=2, order=2):
+               reflect_index=(1, 2), g=2, order=2, b=None, aux_bc_lower=None, aux_bc_upper=None):
     """Dimensionally split step (Godunov: x-sweeps, ghost refill, y-sweeps; no transverse
     propagation) on interior q (meqn, mx, my) -> (qnew, cfl = max of the two sweeps)."""
     meqn, mx, my = q.shape
     dtdx, dtdy = dt / dx, dt / dy
+    bbc = None if b is None else fill_aux_ghosts(b, g, aux_bc_lower or bc_lower, aux_bc_upper or bc_upper)
     qbc = fill_ghosts(q, g, bc_lower, bc_upper, reflect_index)
-    qa, fd, _, _, smx = sweep(qbc[:, :, g:g + my], dtdx, g, solver, phys, mthlim, order, 0)
+    qa, fd, _, _, smx = sweep(qbc[:, :, g:g + my], dtdx, g, solver, phys, mthlim, order, 0,
+                              None if bbc is None else bbc[:, g:g + my])
     q1 = q + qa - dtdx * fd
     qbc = fill_ghosts(q1, g, bc_lower, bc_upper, reflect_index)
     qa, fd, _, _, smy = sweep(_swap_xy(qbc[:, g:g + mx, :], solver), dtdy, g, solver, phys, mthlim,
-                              order, 0)
+                              order, 0, None if bbc is None else bbc[g:g + mx, :].transpose(0, 1))
     q2 = q1 + _swap_xy(qa - dtdy * fd, solver)
     return q2, torch.maximum(dtdx * smx, dtdy * smy)
