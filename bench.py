@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--no-euler", action="store_true")
     p.add_argument("--euler-n", type=int, default=2048)
     p.add_argument("--euler-steps", type=int, default=500)
+    p.add_argument("--no-bathy", action="store_true")
+    p.add_argument("--bathy-steps", type=int, default=500)
     p.add_argument("--no-precision", action="store_true")
     p.add_argument("--precision-sizes", default="4096,8192")
     p.add_argument("--precision-steps", type=int, default=1000)
@@ -192,6 +194,49 @@ def run_euler(n, steps, warmup, seed):
                      "roofline_frac": bpc * n * n * att / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
         del g
     out["workload"] = "euler_4wave_efix_unsplit_mc_extrap_quadrants_seed%d_%dx%d" % (seed, n, n)
+    return out
+
+
+def run_bathy(n, steps, warmup):
+    """Shallow water over bathymetry (f-wave solver, aux plane b): the 2D sill problem on
+    [-1,1]^2 at n x n, g = 9.81, MC, transverse_waves 2, x extrap / y wall, CFL 0.9 device dt;
+    unsplit and split, FP64 and FP32.  Algorithmic bytes per cell-update: q read + q write + the
+    aux plane read, 7 values (56 B FP64, 28 B FP32)."""
+    import torch
+    from clawpack_amd import problems
+    from clawpack_amd.device import DeviceGrid, make_desc
+
+    q, aux, dx, dy = problems.sill(n, n)
+    out = {}
+    for split in (False, True):
+        for dtype_id, name, bpc in ((0, "fp64", 56.0), (1, "fp32", 28.0)):
+            d = make_desc(2, dtype_id, n, n, dx, dy, 3, 3, order=2, transverse_waves=2,
+                          mthlim=(4, 4, 4), bc_lower=(1, 3), bc_upper=(1, 3), grav=9.81,
+                          dry_tolerance=1e-3)
+            g = DeviceGrid(d, "cuda:0", split=split)
+            g.upload(q)
+            g.upload_aux(aux)
+            g.init_ctrl(0.0, 1e-5, target_steps=10 ** 9)
+            g.advance(warmup)
+            torch.cuda.synchronize()
+            c0 = g.read_ctrl()
+            s = torch.cuda.current_stream()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            g.advance(steps)
+            e1.record(s)
+            torch.cuda.synchronize()
+            c1 = g.read_ctrl()
+            assert c1.status == 0, "device status %d" % c1.status
+            acc = int(c1.numsteps - c0.numsteps)
+            att = int(c1.attempts - c0.attempts)
+            ms = e0.elapsed_time(e1)
+            out[("split_" if split else "unsplit_") + name] = {
+                "cell_updates_per_s": n * n * acc / (ms * 1e-3), "ms_per_step": ms / acc,
+                "accepted": acc, "attempts": att, "t": c1.t,
+                "roofline_frac": bpc * n * n * att / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            del g
+    out["workload"] = "swe_bathymetry_fwave_sill_mc_extrap_wall_%dx%d" % (n, n)
     return out
 
 
@@ -403,6 +448,9 @@ def main():
     if not a.no_euler:
         out["euler"] = run_euler(a.euler_n, a.euler_steps, 10, 1)
         progress("euler done")
+    if not a.no_bathy:
+        out["bathymetry"] = run_bathy(a.n, a.bathy_steps, a.warmup)
+        progress("bathymetry done")
     if not a.no_precision:
         out["precision"] = run_precision([int(x) for x in a.precision_sizes.split(",")],
                                          a.precision_steps, a.seed)

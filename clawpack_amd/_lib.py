@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libclawpack_amd.so")
 
-SOLVER_SHALLOW_ROE_EFIX, SOLVER_EULER_4WAVE = 0, 1
+SOLVER_SHALLOW_ROE_EFIX, SOLVER_EULER_4WAVE, SOLVER_SHALLOW_BATHY_FWAVE = 0, 1, 2
 FP64, FP32 = 0, 1
 STATUS_BAD_STATE, STATUS_CFL_FIXED_DT, STATUS_MAX_STEPS = 1, 2, 4
 FLAG_RUN_TO_TEND, FLAG_DT_VARIABLE = 1, 2
@@ -24,6 +24,8 @@ class ClawDesc(C.Structure):
         ("reflect_index", C.c_int32 * 2), ("seg_h", C.c_int32), ("dt_variable", C.c_int32),
         ("dx", C.c_double), ("dy", C.c_double), ("grav", C.c_double), ("gamma", C.c_double),
         ("cfl_max", C.c_double), ("cfl_desired", C.c_double), ("dt_max", C.c_double),
+        ("num_aux", C.c_int32), ("fwave", C.c_int32), ("aux_bc_lower", C.c_int32 * 2),
+        ("aux_bc_upper", C.c_int32 * 2), ("dry_tolerance", C.c_double),
     ]
 
 
@@ -59,6 +61,10 @@ SIGNATURES = {
     "claw2d_advance": (C.c_int, [_PD, _VP, _VP, _VP, C.c_longlong, _VP]),
     "claw2d_step_ds": (C.c_int, [_PD, _VP, _VP, C.c_double, _VP, _VP]),
     "claw2d_advance_ds": (C.c_int, [_PD, _VP, _VP, _VP, C.c_longlong, _VP]),
+    "claw2d_step_aux": (C.c_int, [_PD, _VP, _VP, _VP, C.c_double, _VP, _VP]),
+    "claw2d_advance_aux": (C.c_int, [_PD, _VP, _VP, _VP, _VP, C.c_longlong, _VP]),
+    "claw2d_step_ds_aux": (C.c_int, [_PD, _VP, _VP, _VP, C.c_double, _VP, _VP]),
+    "claw2d_advance_ds_aux": (C.c_int, [_PD, _VP, _VP, _VP, _VP, C.c_longlong, _VP]),
     "claw2d_mass_sum_workspace_bytes": (C.c_size_t, [_PD]),
     "claw2d_mass_sum": (C.c_int, [_PD, _VP, C.c_int32, _VP, _VP, _VP]),
     "claw2d_adjoint_workspace_bytes": (C.c_size_t, [_PD]),

@@ -119,14 +119,26 @@ template <class T, class P> struct StepArgs {
   int fuse;
   int total_waves;
   double cfl_max, cfl_desired, dt_max;
+  // aux plane 0 (the bathymetry of an f-wave policy) and its boundary conditions
+  const T* aux;
+  int abx_lo, abx_hi, aby_lo, aby_hi;
   typename P::Par par;
 };
+
+// Per-cell data of a policy, with its aux field when it reads one.
+template <class P, class T>
+__device__ __forceinline__ typename P::Cell prep_cell(const T q[P::NEQ], T b, const typename P::Par& pr) {
+  if constexpr (P::NAUX > 0) return P::prep(q, b, pr);
+  else return P::prep(q, pr);
+}
 
 // Limits one interface and forms its contributions (interface frame), following flux2.f90:
 //   to the left/lower cell : LX = -r*(amdq + cqxx/2),  LG1/LG2 = rt * (-r/2) * B^-/B^+ (amdq + cqxx)
 //   to the right/upper cell: RX = -r*(apdq - cqxx/2),  RG1/RG2 = rt * (-r/2) * B^-/B^+ (apdq - cqxx)
 // (qadd, fadd += cqxx/2, cqxx folded into amdq/apdq for transverse_waves = 2, gadd), where r is
 // dt/dx along the interface normal and rt the transverse dt/dy that step2.f90 applies to gadd.
+// For an f-wave policy amdq = sum sf_p alpha_p e_p, apdq = sum (1 - sf_p) alpha_p e_p and the
+// correction carries sign(s_p) where the wave form has |s_p| (flux2.f90 with use_fwave).
 // alL/mL and alR/mR are the wave strengths of the interfaces below/above along the sweep and
 // the eigenvector products e_p(neighbour).e_p(this) the limiter needs.
 template <class T, class P, int LIM>
@@ -145,14 +157,15 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
 #pragma unroll
   for (int p = 0; p < NW; ++p) {
     const T s = P::speed(f, p), al = f.al[p], sf = f.sf[p];
-    const T e = sf * al, q = (s - sf) * al;
+    const T e = sf * al, q = P::FWAVE ? al - e : (s - sf) * al;
     T k = T(0);
     if (second) {
       const T aL = alL[p], aR = alR[p], ml = mL[p], mr = mR[p];  // rvalues: no lvalue select
       const T t = (s > T(0) ? aL * ml : aR * mr) * in[p];
       const int meth = LIM > 0 ? LIM : a.mthlim[p];
       const T sa = s < T(0) ? -s : s;
-      k = sa * (T(1) - sa * r) * limited(al, t, meth);
+      const T sw = P::FWAVE ? copysign(T(1), s) : sa;
+      k = sw * (T(1) - sa * r) * limited(al, t, meth);
     }
     XL[p] = -r * fma(T(0.5), k, e);
     XR[p] = -r * fma(T(-0.5), k, q);
@@ -221,6 +234,11 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   int cx;
   bool fx;
   map_index(c, mx, a.bcx_lo, a.bcx_hi, cx, fx);
+  int cxa = cx;  // aux column (aux BCs may differ from the state's)
+  if constexpr (P::NAUX > 0) {
+    bool fa;
+    map_index(c, mx, a.abx_lo, a.abx_hi, cxa, fa);
+  }
   const int addrL = (lane > 0 ? lane - 1 : 0) * 4;
   const int addrR = (lane < 63 ? lane + 1 : 63) * 4;
   const bool own_col = lane >= 2 && lane < 2 + kOwn && c < mx;
@@ -231,7 +249,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   // Loads row R of this lane's column.  The wall sign flips are returned, not applied, so the
   // consumer (one row later) and not the load waits for the data.
   const bool walls = fx || a.bcy_lo == CLAW_BC_WALL || a.bcy_hi == CLAW_BC_WALL;
-  auto load_row = [&](int R, T q[NEQ], bool& fy) {
+  auto load_row = [&](int R, T q[NEQ], bool& fy, T& b) {
     int ry = R;
     fy = false;
     if (R < 0 || R >= my) map_index(R, my, a.bcy_lo, a.bcy_hi, ry, fy);
@@ -239,6 +257,13 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
     const T* src = qin + (size_t)ry * (size_t)mx + (size_t)cx;
 #pragma unroll
     for (int m = 0; m < NEQ; ++m) q[m] = src[(size_t)m * plane];
+    if constexpr (P::NAUX > 0) {
+      int ra = R;
+      bool fa;
+      if (R < 0 || R >= my) map_index(R, my, a.aby_lo, a.aby_hi, ra, fa);
+      ra = __builtin_amdgcn_readfirstlane(ra);
+      b = a.aux[(size_t)ra * (size_t)mx + (size_t)cxa];
+    }
   };
   auto apply_flips = [&](T q[NEQ], bool fy) {
     if (!walls) return;
@@ -267,14 +292,16 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   for (int p = 0; p < NW; ++p) yal2[p] = ymprev[p] = T(0);
 
   bool fnext;
-  load_row(y0 - 2, qnext, fnext);
+  T bnext = T(0);
+  load_row(y0 - 2, qnext, fnext, bnext);
   for (int R = y0 - 2; R <= y1 + 1; ++R) {
     T qR[NEQ];
 #pragma unroll
     for (int m = 0; m < NEQ; ++m) qR[m] = qnext[m];
+    const T bR = bnext;
     apply_flips(qR, fnext);
-    if (R <= y1) load_row(R + 1, qnext, fnext);
-    const Cell cR = P::prep(qR, a.par);
+    if (R <= y1) load_row(R + 1, qnext, fnext, bnext);
+    const Cell cR = prep_cell<P>(qR, bR, a.par);
 
     // ---- request the left neighbour's cell data for the x-interfaces of row R now, so that the
     // y work below hides the ds_bpermute latency
@@ -399,7 +426,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         T qs[NEQ];
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) qs[m] = qR[m] + (RX[m] + shfl(LX[m], addrR));
-        cY = P::prep(qs, a.par);
+        cY = prep_cell<P>(qs, bR, a.par);
       } else {
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) {
@@ -701,9 +728,10 @@ static int choose_seg_h(const claw2d_desc* d, long resident) {
 
 static int check_desc(const claw2d_desc* d) {
   if (!d) return fail(-1, "null descriptor");
-  if (d->solver != CLAW_SOLVER_SHALLOW_ROE_EFIX && d->solver != CLAW_SOLVER_EULER_4WAVE)
+  if (d->solver != CLAW_SOLVER_SHALLOW_ROE_EFIX && d->solver != CLAW_SOLVER_EULER_4WAVE &&
+      d->solver != CLAW_SOLVER_SHALLOW_BATHY_FWAVE)
     return fail(-2, "unsupported solver id");
-  const int neq = d->solver == CLAW_SOLVER_SHALLOW_ROE_EFIX ? 3 : 4;
+  const int neq = d->solver == CLAW_SOLVER_EULER_4WAVE ? 4 : 3;
   if (d->num_eqn != neq || d->num_waves != neq) return fail(-3, "num_eqn/num_waves do not match the solver");
   if (d->dtype != CLAW_FP64 && d->dtype != CLAW_FP32) return fail(-4, "dtype must be FP64 or FP32");
   if (d->mx < 2 || d->my < 2) return fail(-5, "mx and my must be >= 2");
@@ -721,6 +749,21 @@ static int check_desc(const claw2d_desc* d) {
   if (!(d->dx > 0) || !(d->dy > 0)) return fail(-13, "dx, dy must be positive");
   if (d->seg_h < 0) return fail(-14, "seg_h must be >= 0");
   if ((long long)d->mx * d->my > (1LL << 31)) return fail(-15, "grid too large for 32-bit row index math");
+  if (d->solver == CLAW_SOLVER_SHALLOW_BATHY_FWAVE) {
+    if (d->num_aux < 1) return fail(-16, "shallow_bathymetry_fwave needs num_aux >= 1 (aux plane 0 = bathymetry)");
+    if (d->fwave != 1) return fail(-17, "shallow_bathymetry_fwave returns f-waves: fwave must be 1");
+    if (!(d->dry_tolerance >= 0)) return fail(-18, "dry_tolerance must be >= 0");
+    for (int k = 0; k < 2; ++k) {
+      const int lo = d->aux_bc_lower[k], hi = d->aux_bc_upper[k];
+      if (lo < 1 || lo > 3 || hi < 1 || hi > 3)
+        return fail(-19, "aux bc must be extrap(1), periodic(2) or wall(3)");
+      if ((lo == CLAW_BC_PERIODIC) != (hi == CLAW_BC_PERIODIC))
+        return fail(-11, "periodic aux bc must be set on both sides");
+    }
+  } else {
+    if (d->num_aux != 0) return fail(-16, "this Riemann solver takes no aux arrays (num_aux must be 0)");
+    if (d->fwave != 0) return fail(-17, "fwave = 1 needs an f-wave Riemann solver (shallow_bathymetry_fwave)");
+  }
   return 0;
 }
 
@@ -742,12 +785,24 @@ static void fill_args(StepArgs<T, P>& a, const claw2d_desc* d, long resident) {
   a.order = d->order;
   a.tw = d->transverse_waves;
   for (int p = 0; p < 4; ++p) a.mthlim[p] = d->mthlim[p];
+  a.aux = nullptr;
+  a.abx_lo = d->aux_bc_lower[0];
+  a.abx_hi = d->aux_bc_upper[0];
+  a.aby_lo = d->aux_bc_lower[1];
+  a.aby_hi = d->aux_bc_upper[1];
 }
 
 template <class T> static typename SweP<T>::Par swe_par(const claw2d_desc* d) {
   typename SweP<T>::Par p;
   p.g = T(d->grav);
   p.sqg = T(std::sqrt(d->grav));
+  return p;
+}
+template <class T> static typename SweBathyP<T>::Par bathy_par(const claw2d_desc* d) {
+  typename SweBathyP<T>::Par p;
+  p.g = T(d->grav);
+  p.hg = T(0.5 * d->grav);
+  p.tol = T(d->dry_tolerance);
   return p;
 }
 template <class T> static typename EulerP<T>::Par euler_par(const claw2d_desc* d) {
@@ -759,7 +814,7 @@ template <class T> static typename EulerP<T>::Par euler_par(const claw2d_desc* d
 
 template <class T, class P, bool SPLIT>
 static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q0, void* q1,
-                       claw2d_ctrl* ctrl, int use_ctrl, double dt, hipStream_t s) {
+                       const void* aux, claw2d_ctrl* ctrl, int use_ctrl, double dt, hipStream_t s) {
   bool same = true;
   for (int p = 1; p < P::NW; ++p) same = same && d->mthlim[p] == d->mthlim[0];
   const int lim = same ? d->mthlim[0] : -1;
@@ -782,6 +837,7 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   a.cfl_desired = d->cfl_desired;
   a.dt_max = d->dt_max;
   if (SPLIT) a.tw = 0;  // dimensional splitting propagates no transverse waves (method(3) < 0)
+  a.aux = (const T*)aux;
   const int waves = a.nstrips * a.nsegs;
   a.total_waves = waves;
   const int blocks = (waves + kWPB - 1) / kWPB;
@@ -795,16 +851,24 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   return e == hipSuccess ? 0 : hip_fail(e, "step2_fused launch");
 }
 
-template <bool SPLIT>
-static int dispatch_step(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, int use_ctrl,
-                         double dt, hipStream_t s) {
-  const bool swe = d->solver == CLAW_SOLVER_SHALLOW_ROE_EFIX;
-  if (d->dtype == CLAW_FP64) {
-    return swe ? launch_step<double, SweP<double>, SPLIT>(d, swe_par<double>(d), q0, q1, ctrl, use_ctrl, dt, s)
-               : launch_step<double, EulerP<double>, SPLIT>(d, euler_par<double>(d), q0, q1, ctrl, use_ctrl, dt, s);
+template <class T, bool SPLIT>
+static int dispatch_typed(const claw2d_desc* d, void* q0, void* q1, const void* aux, claw2d_ctrl* ctrl,
+                          int use_ctrl, double dt, hipStream_t s) {
+  switch (d->solver) {
+    case CLAW_SOLVER_SHALLOW_ROE_EFIX:
+      return launch_step<T, SweP<T>, SPLIT>(d, swe_par<T>(d), q0, q1, aux, ctrl, use_ctrl, dt, s);
+    case CLAW_SOLVER_SHALLOW_BATHY_FWAVE:
+      return launch_step<T, SweBathyP<T>, SPLIT>(d, bathy_par<T>(d), q0, q1, aux, ctrl, use_ctrl, dt, s);
+    default:
+      return launch_step<T, EulerP<T>, SPLIT>(d, euler_par<T>(d), q0, q1, aux, ctrl, use_ctrl, dt, s);
   }
-  return swe ? launch_step<float, SweP<float>, SPLIT>(d, swe_par<float>(d), q0, q1, ctrl, use_ctrl, dt, s)
-             : launch_step<float, EulerP<float>, SPLIT>(d, euler_par<float>(d), q0, q1, ctrl, use_ctrl, dt, s);
+}
+
+template <bool SPLIT>
+static int dispatch_step(const claw2d_desc* d, void* q0, void* q1, const void* aux, claw2d_ctrl* ctrl,
+                         int use_ctrl, double dt, hipStream_t s) {
+  return d->dtype == CLAW_FP64 ? dispatch_typed<double, SPLIT>(d, q0, q1, aux, ctrl, use_ctrl, dt, s)
+                               : dispatch_typed<float, SPLIT>(d, q0, q1, aux, ctrl, use_ctrl, dt, s);
 }
 
 template <class T, class P>
@@ -872,6 +936,7 @@ static int check_batch(const claw2d_desc* d, const void* grids, int32_t ngrids, 
                        const void* q0, const void* q1, const void* ctrls, const void* counter) {
   int rc = check_desc(d);
   if (rc) return rc;
+  if (d->num_aux > 0) return fail(-25, "batched grids take no aux arrays");
   if (!grids || !q0 || !q1 || !ctrls || !counter) return fail(-22, "null buffer");
   if (q0 == q1) return fail(-23, "q0 and q1 must differ");
   if (ngrids < 1) return fail(-30, "ngrids must be >= 1");
@@ -918,52 +983,80 @@ int claw2d_ctrl_init(const claw2d_desc* d, claw2d_ctrl* c, double t0, double dt0
   return 0;
 }
 
-int claw2d_step(const claw2d_desc* d, const void* q_in, void* q_out, double dt, claw2d_ctrl* ctrl,
-                void* stream) {
+}  // extern "C"
+
+namespace claw {
+// Shared bodies of the step / advance entry points (with_aux: the _aux variants).
+template <bool SPLIT>
+static int step_entry(const claw2d_desc* d, const void* q_in, void* q_out, const void* aux, double dt,
+                      claw2d_ctrl* ctrl, void* stream, bool with_aux, const char* name) {
   int rc = check_desc(d);
   if (rc) return rc;
   if (!q_in || !q_out || !ctrl) return fail(-22, "null buffer");
   if (q_in == q_out) return fail(-23, "q_in and q_out must differ");
   if (!(dt > 0)) return fail(-21, "dt must be positive");
-  hipStream_t s = (hipStream_t)stream;
-  return dispatch_step<false>(d, const_cast<void*>(q_in), q_out, ctrl, 0, dt, s);
+  if (!with_aux && d->num_aux > 0) return fail(-26, std::string("num_aux > 0: use ") + name + "_aux");
+  if (with_aux && d->num_aux > 0 && !aux) return fail(-22, "null aux buffer");
+  return dispatch_step<SPLIT>(d, const_cast<void*>(q_in), q_out, aux, ctrl, 0, dt, (hipStream_t)stream);
+}
+
+template <bool SPLIT>
+static int advance_entry(const claw2d_desc* d, void* q0, void* q1, const void* aux, claw2d_ctrl* ctrl,
+                         long long attempts, void* stream, bool with_aux, const char* name) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!q0 || !q1 || !ctrl) return fail(-22, "null buffer");
+  if (q0 == q1) return fail(-23, "q0 and q1 must differ");
+  if (!with_aux && d->num_aux > 0) return fail(-26, std::string("num_aux > 0: use ") + name + "_aux");
+  if (with_aux && d->num_aux > 0 && !aux) return fail(-22, "null aux buffer");
+  for (long long k = 0; k < attempts; ++k) {
+    rc = dispatch_step<SPLIT>(d, q0, q1, aux, ctrl, 1, 0.0, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return 0;
+}
+}  // namespace claw
+
+extern "C" {
+
+int claw2d_step(const claw2d_desc* d, const void* q_in, void* q_out, double dt, claw2d_ctrl* ctrl,
+                void* stream) {
+  return step_entry<false>(d, q_in, q_out, nullptr, dt, ctrl, stream, false, "claw2d_step");
 }
 
 int claw2d_advance(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, long long attempts,
                    void* stream) {
-  int rc = check_desc(d);
-  if (rc) return rc;
-  if (!q0 || !q1 || !ctrl) return fail(-22, "null buffer");
-  if (q0 == q1) return fail(-23, "q0 and q1 must differ");
-  hipStream_t s = (hipStream_t)stream;
-  for (long long k = 0; k < attempts; ++k) {
-    rc = dispatch_step<false>(d, q0, q1, ctrl, 1, 0.0, s);
-    if (rc) return rc;
-  }
-  return 0;
+  return advance_entry<false>(d, q0, q1, nullptr, ctrl, attempts, stream, false, "claw2d_advance");
 }
 
 int claw2d_step_ds(const claw2d_desc* d, const void* q_in, void* q_out, double dt, claw2d_ctrl* ctrl,
                    void* stream) {
-  int rc = check_desc(d);
-  if (rc) return rc;
-  if (!q_in || !q_out || !ctrl) return fail(-22, "null buffer");
-  if (q_in == q_out) return fail(-23, "q_in and q_out must differ");
-  if (!(dt > 0)) return fail(-21, "dt must be positive");
-  return dispatch_step<true>(d, const_cast<void*>(q_in), q_out, ctrl, 0, dt, (hipStream_t)stream);
+  return step_entry<true>(d, q_in, q_out, nullptr, dt, ctrl, stream, false, "claw2d_step_ds");
 }
 
 int claw2d_advance_ds(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, long long attempts,
                       void* stream) {
-  int rc = check_desc(d);
-  if (rc) return rc;
-  if (!q0 || !q1 || !ctrl) return fail(-22, "null buffer");
-  if (q0 == q1) return fail(-23, "q0 and q1 must differ");
-  for (long long k = 0; k < attempts; ++k) {
-    rc = dispatch_step<true>(d, q0, q1, ctrl, 1, 0.0, (hipStream_t)stream);
-    if (rc) return rc;
-  }
-  return 0;
+  return advance_entry<true>(d, q0, q1, nullptr, ctrl, attempts, stream, false, "claw2d_advance_ds");
+}
+
+int claw2d_step_aux(const claw2d_desc* d, const void* q_in, void* q_out, const void* aux, double dt,
+                    claw2d_ctrl* ctrl, void* stream) {
+  return step_entry<false>(d, q_in, q_out, aux, dt, ctrl, stream, true, "claw2d_step");
+}
+
+int claw2d_advance_aux(const claw2d_desc* d, void* q0, void* q1, const void* aux, claw2d_ctrl* ctrl,
+                       long long attempts, void* stream) {
+  return advance_entry<false>(d, q0, q1, aux, ctrl, attempts, stream, true, "claw2d_advance");
+}
+
+int claw2d_step_ds_aux(const claw2d_desc* d, const void* q_in, void* q_out, const void* aux, double dt,
+                       claw2d_ctrl* ctrl, void* stream) {
+  return step_entry<true>(d, q_in, q_out, aux, dt, ctrl, stream, true, "claw2d_step_ds");
+}
+
+int claw2d_advance_ds_aux(const claw2d_desc* d, void* q0, void* q1, const void* aux, claw2d_ctrl* ctrl,
+                          long long attempts, void* stream) {
+  return advance_entry<true>(d, q0, q1, aux, ctrl, attempts, stream, true, "claw2d_advance_ds");
 }
 
 size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc*) { return 2 * kSumBlocks * sizeof(double); }

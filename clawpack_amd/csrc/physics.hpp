@@ -13,8 +13,12 @@
 // right eigenvectors (for both systems M has entries 1/2, +-a, +-1/(2a) and an identity block).
 // The results equal upstream's component-wise evaluation up to rounding.
 //
-//   SweP   <- riemann/src/rpn2_shallow_roe_with_efix.f90, rpt2_shallow_roe_with_efix.f90
-//   EulerP <- riemann/src/rpn2_euler_4wave.f90, rpt2_euler.f90
+//   SweP      <- riemann/src/rpn2_shallow_roe_with_efix.f90, rpt2_shallow_roe_with_efix.f90
+//   EulerP    <- riemann/src/rpn2_euler_4wave.f90, rpt2_euler.f90
+//   SweBathyP <- riemann/src/rpn2_shallow_bathymetry_fwave.f90, rpt2_shallow_bathymetry_fwave.f90
+//                (f-waves: alpha_p are f-wave strengths, sf_p the left-going fraction 1, 1/2 or 0,
+//                 amdq = sum sf_p alpha_p e_p, apdq = sum (1 - sf_p) alpha_p e_p, and the
+//                 correction uses sign(s_p) in place of |s_p|)
 // (upstream riemann submodule; empty in the reference tree, .gitmodules:10-12).
 //
 // Limiter (upstream limiter.f90/philim): theta_p = W_p(upwind).W_p / W_p.W_p.  With
@@ -96,6 +100,8 @@ template <class T> __device__ __forceinline__ T limited(T al, T t, int meth) {
 // ------------------------------------------------------------------------------------------
 template <class T> struct SweP {
   static constexpr int NEQ = 3, NW = 3, NC = 7, NL = 6;
+  static constexpr int NAUX = 0;        // aux fields read per cell
+  static constexpr bool FWAVE = false;  // waves (not f-waves)
   // e_p(A).e_p(B) is identically 1 for the shear wave
   __device__ __forceinline__ static constexpr bool unit_m(int p) { return p == 1; }
   struct Par { T g, sqg; };
@@ -236,6 +242,8 @@ template <class T> struct SweP {
 // ------------------------------------------------------------------------------------------
 template <class T> struct EulerP {
   static constexpr int NEQ = 4, NW = 4, NC = 9, NL = 8;
+  static constexpr int NAUX = 0;
+  static constexpr bool FWAVE = false;
   __device__ __forceinline__ static constexpr bool unit_m(int) { return false; }
   struct Par { T gamma, gamma1; };
   // per-cell: q, 1/sqrt(rho), rho u/sqrt(rho), rho v/sqrt(rho), (E+p)/sqrt(rho), c
@@ -407,6 +415,90 @@ template <class T> struct EulerP {
     Gp[1] = fma(Gp[0], f.u, c2p);
     Gp[2] = fma(c3, f.v + f.a, c1p * f.v);
     Gp[3] = fma(c3, f.H + f.v * f.a, fma(c2p, f.u, c1p * k));
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+// Shallow water over bathymetry b = aux(1), f-wave form (Bale, LeVeque, Mitran, Rossmanith 2002).
+// The jump delta = f(q_r) - f(q_l) + (0, g hbar (b_r - b_l), 0) is split along SweP's Roe
+// eigenvectors e1 = (1, u-a, v), e2 = (0, 0, 1), e3 = (1, u+a, v) into f-wave strengths
+//   beta1 = (s3 d1 - d2)/(2a),  beta2 = d3 - v d1,  beta3 = (d2 - s1 d1)/(2a),
+// with Roe speeds; the limiter, the transverse split and the eigenvector algebra are SweP's.
+// A cell with h <= dry_tolerance has zero velocity (P = Q = 0); an interface between two dry
+// cells has no waves (u = v = a = 0 there, which also keeps the transverse algebra finite).
+// Flux terms from the per-cell P = hu/sqrt(h), Q = hv/sqrt(h), s = sqrt(h):
+//   hu = s P,  hu^2 = P^2,  huv = P Q.
+// ------------------------------------------------------------------------------------------
+template <class T> struct SweBathyP : SweP<T> {
+  static constexpr int NAUX = 1;
+  static constexpr bool FWAVE = true;
+  static constexpr int NC = 7;
+  struct Par { T g, hg, tol; };  // g, g/2, dry_tolerance
+  // per-cell: q, sqrt(h), hu/sqrt(h), hv/sqrt(h) (zero when dry), bathymetry
+  struct Cell { T q[3]; T s, P, Q, b; };
+  using Iface = typename SweP<T>::Iface;
+
+  // depth 0 is a dry cell, a negative depth (or NaN) is a bad state
+  __device__ __forceinline__ static bool bad(const T q[3]) { return !(q[0] >= T(0)); }
+
+  __device__ __forceinline__ static Cell prep(const T q[3], T b, const Par& pr) {
+    Cell c;
+    c.q[0] = q[0]; c.q[1] = q[1]; c.q[2] = q[2];
+    c.b = b;
+    // clamped away from 0 so that a dry cell (h = 0) gets s = 0 without a 0 * inf
+    const T rs = frsq(vmax(q[0], T(1e-30)));
+    c.s = q[0] * rs;
+    const bool wet = q[0] > pr.tol;
+    c.P = wet ? q[1] * rs : T(0);
+    c.Q = wet ? q[2] * rs : T(0);
+    return c;
+  }
+  __device__ __forceinline__ static void pack(const Cell& c, T o[NC]) {
+    o[0] = c.q[0]; o[1] = c.q[1]; o[2] = c.q[2]; o[3] = c.s; o[4] = c.P; o[5] = c.Q; o[6] = c.b;
+  }
+  __device__ __forceinline__ static Cell unpack(const T o[NC]) {
+    Cell c;
+    c.q[0] = o[0]; c.q[1] = o[1]; c.q[2] = o[2]; c.s = o[3]; c.P = o[4]; c.Q = o[5]; c.b = o[6];
+    return c;
+  }
+
+  // left-going fraction of an f-wave of speed s
+  __device__ __forceinline__ static T left_frac(T s) {
+    return s < T(0) ? T(1) : (s > T(0) ? T(0) : T(0.5));
+  }
+
+  template <int DIR>
+  __device__ __forceinline__ static void rpn2(const Cell& L, const Cell& R, const Par& pr, Iface& f) {
+    const T pnL = DIR == 0 ? L.P : L.Q, ptL = DIR == 0 ? L.Q : L.P;
+    const T pnR = DIR == 0 ? R.P : R.Q, ptR = DIR == 0 ? R.Q : R.P;
+    const bool wet = L.q[0] > pr.tol || R.q[0] > pr.tol;
+    const T inv = wet ? frcp(L.s + R.s) : T(0);
+    const T u = (pnL + pnR) * inv;
+    const T v = (ptL + ptR) * inv;
+    const T hb = T(0.5) * (L.q[0] + R.q[0]);
+    const T gh = wet ? pr.g * hb : T(1);
+    const T ra = frsq(gh);
+    const T a = wet ? gh * ra : T(0);
+    const T ha = wet ? T(0.5) * ra : T(0);
+    // Equal states must give exactly zero f-waves (an extrap ghost interface): a contracted
+    // fma(a, b, -(c*d)) with a*b == c*d leaves the rounding error of c*d, and beam-warming turns
+    // any nonzero wave, however small, into a full upwind wave.  So both products are rounded.
+    T d1, d3;
+    {
+#pragma clang fp contract(off)
+      d1 = R.s * pnR - L.s * pnL;
+      d3 = pnR * ptR - pnL * ptL;
+    }
+    const T d2 = (fma(pnR, pnR, pr.hg * (R.q[0] * R.q[0])) - fma(pnL, pnL, pr.hg * (L.q[0] * L.q[0]))) +
+                 (pr.g * hb) * (R.b - L.b);
+    const T s1 = u - a, s3 = u + a;
+    f.al[0] = (s3 * d1 - d2) * ha;
+    f.al[1] = d3 - v * d1;
+    f.al[2] = (d2 - s1 * d1) * ha;
+    f.sf[0] = left_frac(s1);
+    f.sf[1] = left_frac(u);
+    f.sf[2] = left_frac(s3);
+    f.u = u; f.v = v; f.a = a; f.ha = ha;
   }
 };
 

@@ -23,8 +23,19 @@ def _stream_handle(stream=None):
 def make_desc(solver_id, dtype, mx, my, dx, dy, num_eqn, num_waves, order=2, transverse_waves=2,
               mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_upper=(1, 1), reflect_index=(1, 2),
               grav=1.0, gamma=1.4, cfl_max=1.0, cfl_desired=0.9, dt_max=1e99, dt_variable=True,
-              seg_h=0, num_ghost=2):
+              seg_h=0, num_ghost=2, num_aux=None, fwave=None, aux_bc_lower=None, aux_bc_upper=None,
+              dry_tolerance=1e-3):
+    """num_aux and fwave default to what the solver needs (one aux field, the bathymetry, and
+    f-waves for shallow_bathymetry_fwave); aux BCs default to the state's."""
     d = ClawDesc()
+    bathy = int(solver_id) == _lib.SOLVER_SHALLOW_BATHY_FWAVE
+    d.num_aux = int((1 if bathy else 0) if num_aux is None else num_aux)
+    d.fwave = int(bool(bathy if fwave is None else fwave))
+    d.dry_tolerance = float(dry_tolerance)
+    abl = bc_lower if aux_bc_lower is None else aux_bc_lower
+    abu = bc_upper if aux_bc_upper is None else aux_bc_upper
+    for k in range(2):
+        d.aux_bc_lower[k], d.aux_bc_upper[k] = int(abl[k]), int(abu[k])
     d.solver, d.dtype, d.mx, d.my = int(solver_id), int(dtype), int(mx), int(my)
     d.num_eqn, d.num_waves, d.num_ghost = int(num_eqn), int(num_waves), int(num_ghost)
     d.order, d.transverse_waves = int(order), int(transverse_waves)
@@ -42,8 +53,9 @@ def make_desc(solver_id, dtype, mx, my, dx, dy, num_eqn, num_waves, order=2, tra
 
 
 class DeviceGrid:
-    """One grid on one GPU: q ping-pong pair, control block, mass-sum workspace.  split=True
-    steps with dimensional splitting (step2ds) instead of the unsplit step2."""
+    """One grid on one GPU: q ping-pong pair, control block, mass-sum workspace, and the aux
+    planes (num_aux, my, mx) when the solver reads aux.  split=True steps with dimensional
+    splitting (step2ds) instead of the unsplit step2."""
 
     def __init__(self, desc, device="cuda", split=False):
         self.lib = _lib.load()
@@ -63,12 +75,34 @@ class DeviceGrid:
         self.mass_out = torch.zeros(1, dtype=torch.float64, device=self.device)
         self.cur = 0
         self.split = bool(split)
+        self.aux = torch.zeros((desc.num_aux, desc.my, desc.mx), dtype=self.torch_dtype,
+                               device=self.device) if desc.num_aux > 0 else None
 
     def _step_fn(self):
-        return self.lib.claw2d_step_ds if self.split else self.lib.claw2d_step
+        """(q_in, q_out, dt, ctrl, stream) -> status, with the aux planes bound when present."""
+        if self.aux is None:
+            return self.lib.claw2d_step_ds if self.split else self.lib.claw2d_step
+        fn = self.lib.claw2d_step_ds_aux if self.split else self.lib.claw2d_step_aux
+        aux = C.c_void_p(self.aux.data_ptr())
+        return lambda d, qi, qo, dt, ctrl, s: fn(d, qi, qo, aux, dt, ctrl, s)
 
     def _advance_fn(self):
-        return self.lib.claw2d_advance_ds if self.split else self.lib.claw2d_advance
+        if self.aux is None:
+            return self.lib.claw2d_advance_ds if self.split else self.lib.claw2d_advance
+        fn = self.lib.claw2d_advance_ds_aux if self.split else self.lib.claw2d_advance_aux
+        aux = C.c_void_p(self.aux.data_ptr())
+        return lambda d, q0, q1, ctrl, n, s: fn(d, q0, q1, aux, ctrl, n, s)
+
+    def upload_aux(self, aux_host):
+        """aux_host: pyclaw layout (num_aux, mx, my) -> device planes (num_aux, my, mx)."""
+        if self.aux is None:
+            raise ValueError("this solver takes no aux arrays")
+        a = torch.from_numpy(np.ascontiguousarray(np.asarray(aux_host, dtype=np.float64)))
+        self.aux.copy_(a.permute(0, 2, 1).contiguous().to(self.torch_dtype))
+
+    def set_aux_planes(self, planes):
+        """planes: device or host tensor (num_aux, my, mx)."""
+        self.aux.copy_(planes)
 
     # ---- control block ------------------------------------------------------------------
     @property

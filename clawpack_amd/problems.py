@@ -100,3 +100,40 @@ def inversion_workload(mx=512, my=512, seed=4, n_humps=8, n_gauges=64, lower=-2.
     rng = np.random.default_rng(seed + 1000)
     gauges = np.sort(rng.choice(mx * my, size=n_gauges, replace=False))
     return eta, gauges, dx, dy
+
+
+def sill(mx=200, my=200, lower=-1.0, upper=1.0, eta_amp=0.1, sill_amp=0.8, depth=1.0):
+    """Shallow water over a sill, in the style of pyclaw's shallow_2d bathymetry example, made
+    two-dimensional: bathymetry b = sill_amp exp(-(x^2 + (y/2)^2)/0.2^2) - depth, a surface hump
+    eta = eta_amp exp(-((x+0.4)^2 + (y-0.2)^2)/0.2^2) on sea level 0, at rest; h = eta - b.
+    Returns (q (3, mx, my), aux (1, mx, my), dx, dy)."""
+    xc, dx = centers(lower, upper, mx)
+    yc, dy = centers(lower, upper, my)
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    b = sill_amp * np.exp(-(X ** 2 + (Y / 2.0) ** 2) / 0.04) - depth
+    eta = eta_amp * np.exp(-((X + 0.4) ** 2 + (Y - 0.2) ** 2) / 0.04)
+    q = np.zeros((3, mx, my), order="F")
+    q[0] = np.maximum(eta - b, 0.0)
+    aux = np.asfortranarray(b[None])
+    return q, aux, dx, dy
+
+
+def random_bathymetry(mx, my, seed, dry_patch=False, amp=0.3):
+    """Random moving water over smooth random bathymetry (wet everywhere unless dry_patch, which
+    empties a block of cells and sets one cell below the usual dry tolerance 1e-3).  Returns
+    (q (3, mx, my), aux (1, mx, my))."""
+    rng = np.random.default_rng(seed)
+    xc = (np.arange(mx) + 0.5) / mx
+    yc = (np.arange(my) + 0.5) / my
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    b = -1.0 + 0.3 * np.exp(-30.0 * ((X - rng.uniform(0.3, 0.7)) ** 2 + (Y - rng.uniform(0.3, 0.7)) ** 2)) \
+        + 0.1 * np.sin(2 * np.pi * (X + 2 * Y + rng.uniform()))
+    q = np.zeros((3, mx, my), order="F")
+    q[0] = 0.1 * rng.random((mx, my)) - b
+    q[1] = amp * rng.standard_normal((mx, my)) * q[0]
+    q[2] = amp * rng.standard_normal((mx, my)) * q[0]
+    if dry_patch:
+        i0, j0 = mx // 4, my // 3
+        q[:, i0:i0 + 3, j0:j0 + 4] = 0.0
+        q[0, mx // 2, my // 2] = 5e-4
+    return q, np.asfortranarray(b[None])

@@ -22,8 +22,9 @@ class ClawSolver2D(Solver):
 
     Supported on the HIP path: dimensional_split True (step2ds, Godunov x-then-y splitting, the
     default) or False (unsplit step2 with transverse_waves 0/1/2), order 1 or 2, limiters 0-5 per
-    wave, BC extrap/periodic/wall, the Riemann plugins shallow_roe_with_efix_2D and
-    euler_4wave_2D, solver.precision 'fp64' (default) or 'fp32'.
+    wave, BC extrap/periodic/wall, the Riemann plugins shallow_roe_with_efix_2D,
+    euler_4wave_2D and shallow_bathymetry_fwave_2D (with fwave = True, state.aux[0] = bathymetry
+    and solver.aux_bc_lower/upper), solver.precision 'fp64' (default) or 'fp32'.
     """
 
     def __init__(self, riemann_solver=None, claw_package=None):
@@ -60,10 +61,28 @@ class ClawSolver2D(Solver):
         if self.rp is None or not hasattr(self.rp, "kernel_id"):
             raise NotImplementedError("Riemann solver not available on the HIP path: use "
                                       "riemann.shallow_roe_with_efix_2D or riemann.euler_4wave_2D")
-        if self.fwave:
-            raise NotImplementedError("fwave=True is not supported on the HIP path")
-        if state.num_aux > 0 or state.index_capa >= 0:
-            raise NotImplementedError("aux arrays / capacity are not supported on the HIP path")
+        rp_fwave = bool(getattr(self.rp, "fwave", False))
+        if bool(self.fwave) != rp_fwave:
+            raise ValueError("solver.fwave must be %s for %s (its Riemann solver returns %s)"
+                             % (rp_fwave, self.rp.__name__, "f-waves" if rp_fwave else "waves"))
+        rp_aux = int(getattr(self.rp, "num_aux", 0))
+        if state.index_capa >= 0:
+            raise NotImplementedError("a capacity function (index_capa) is not supported on the HIP path")
+        if rp_aux == 0 and state.num_aux > 0:
+            raise NotImplementedError("%s reads no aux arrays; aux is only supported with "
+                                      "shallow_bathymetry_fwave_2D on the HIP path" % self.rp.__name__)
+        if state.num_aux < rp_aux:
+            raise ValueError("%s needs num_aux >= %d (aux[0] = bathymetry)" % (self.rp.__name__, rp_aux))
+        if state.num_aux > 0:
+            for k in range(2):
+                lo, hi = self.aux_bc_lower[k], self.aux_bc_upper[k]
+                if lo is None or hi is None:
+                    raise Exception("Lower and upper aux boundary conditions must be specified for "
+                                    "every dimension (solver.aux_bc_lower / solver.aux_bc_upper)")
+                if BC.custom in (lo, hi):
+                    raise NotImplementedError("BC.custom for aux is not supported on the HIP path")
+                if (lo == BC.periodic) != (hi == BC.periodic):
+                    raise Exception("Periodic aux boundary conditions must be set on both sides")
         if self.num_ghost != 2:
             raise NotImplementedError("num_ghost must be 2")
         if self.before_step is not None or self.step_source is not None:
@@ -99,7 +118,10 @@ class ClawSolver2D(Solver):
             bc_lower=self.bc_lower, bc_upper=self.bc_upper, reflect_index=self.reflect_index,
             grav=pd.get("grav", 1.0), gamma=pd.get("gamma", 1.4), cfl_max=self.cfl_max,
             cfl_desired=self.cfl_desired, dt_max=self.dt_max, dt_variable=self.dt_variable,
-            seg_h=self.seg_h)
+            seg_h=self.seg_h, num_aux=state.num_aux, fwave=bool(self.fwave),
+            aux_bc_lower=[b if b is not None else 1 for b in self.aux_bc_lower],
+            aux_bc_upper=[b if b is not None else 1 for b in self.aux_bc_upper],
+            dry_tolerance=pd.get("dry_tolerance", 0.0))
 
     def setup(self, solution):
         state = solution.state
@@ -115,13 +137,15 @@ class ClawSolver2D(Solver):
         if self.autotune and self.seg_h == 0:
             from .. import autotune
             desc.seg_h = autotune.tuned_seg_h(desc)
-        key = (desc.mx, desc.my, desc.num_eqn, desc.dtype)
+        key = (desc.mx, desc.my, desc.num_eqn, desc.dtype, desc.num_aux)
         if self._grid is None or self._desc_key != key:
             self._grid = DeviceGrid(desc)
             self._desc_key = key
         else:
             self._grid.desc = desc
         self._grid.split = bool(self.dimensional_split)
+        if state.num_aux > 0:
+            self._grid.upload_aux(state.aux)  # aux is read-only on the device; refreshed per call
         return self._grid
 
     # ----------------------------------------------------------------------------- steps ---

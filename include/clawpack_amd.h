@@ -34,8 +34,10 @@
 extern "C" {
 #endif
 
-/* Riemann solvers: riemann.shallow_roe_with_efix_2D, riemann.euler_4wave_2D */
-enum { CLAW_SOLVER_SHALLOW_ROE_EFIX = 0, CLAW_SOLVER_EULER_4WAVE = 1 };
+/* Riemann solvers: riemann.shallow_roe_with_efix_2D, riemann.euler_4wave_2D,
+ * riemann.shallow_bathymetry_fwave_2D (f-waves, aux(1) = bathymetry) */
+enum { CLAW_SOLVER_SHALLOW_ROE_EFIX = 0, CLAW_SOLVER_EULER_4WAVE = 1,
+       CLAW_SOLVER_SHALLOW_BATHY_FWAVE = 2 };
 enum { CLAW_FP64 = 0, CLAW_FP32 = 1 };
 /* pyclaw BC codes: BC.custom=0, extrap=1, periodic=2, wall=3 (custom is rejected) */
 enum { CLAW_BC_CUSTOM = 0, CLAW_BC_EXTRAP = 1, CLAW_BC_PERIODIC = 2, CLAW_BC_WALL = 3 };
@@ -64,6 +66,15 @@ typedef struct claw2d_desc {
   double grav;              /* problem_data['grav'] (shallow water) */
   double gamma;             /* problem_data['gamma'] (Euler) */
   double cfl_max, cfl_desired, dt_max;
+  /* aux arrays (method(7) = num_aux) and f-waves (solver.fwave).  Both are required by, and only
+   * accepted with, CLAW_SOLVER_SHALLOW_BATHY_FWAVE: num_aux >= 1 (aux plane 0 is the bathymetry
+   * b), fwave = 1.  aux_bc_* are solver.aux_bc_lower/upper (extrap, periodic or wall; a wall
+   * mirrors aux without a sign change). */
+  int32_t num_aux;
+  int32_t fwave;
+  int32_t aux_bc_lower[2];
+  int32_t aux_bc_upper[2];
+  double dry_tolerance;     /* problem_data['dry_tolerance']: h <= dry_tolerance has zero velocity */
 } claw2d_desc;
 
 /* Device-resident time-loop control block (one per grid). */
@@ -96,7 +107,8 @@ int claw2d_ctrl_init(const claw2d_desc* d, claw2d_ctrl* host_ctrl, double t0, do
 /* One step2 with a caller-chosen dt: q_in_dev -> q_out_dev (the f2py step2 call).  The CFL number
  * max(dt/dx*max|s_x|, dt/dy*max|s_y|) of the step is written to ctrl_dev->cfl, the speeds to
  * ctrl_dev->smax_last, and CLAW_STATUS_BAD_STATE is or-ed into ctrl_dev->status when a depth or
- * density is not positive.  ctrl_dev->smax_bits must be zero on entry (claw2d_ctrl_init and every
+ * density is not positive (with bathymetry: a depth is negative; dry cells have depth 0) or NaN.
+ * ctrl_dev->smax_bits must be zero on entry (claw2d_ctrl_init and every
  * call leave it zero).  Replaces classic2.step2 (upstream classic/step2.f90). */
 int claw2d_step(const claw2d_desc* d, const void* q_in_dev, void* q_out_dev, double dt,
                 claw2d_ctrl* ctrl_dev, void* stream);
@@ -121,6 +133,21 @@ int claw2d_step_ds(const claw2d_desc* d, const void* q_in_dev, void* q_out_dev, 
                    claw2d_ctrl* ctrl_dev, void* stream);
 int claw2d_advance_ds(const claw2d_desc* d, void* q0_dev, void* q1_dev, claw2d_ctrl* ctrl_dev,
                       long long attempts, void* stream);
+
+/* The four entry points above with aux arrays, for solvers with num_aux > 0 (the entry points
+ * without aux reject them).  aux_dev: num_aux planes of my rows of mx cells, x fastest, element
+ * type as the state (the transpose of pyclaw's state.aux(num_aux, mx, my)); read only.  aux ghost
+ * values come from aux_bc_lower/upper by index mapping, as pyclaw's _apply_bcs fills auxbc.
+ * Replace classic2.step2 / step2ds called with auxbc and use_fwave (upstream classic/step2.f90,
+ * step2ds.f90, flux2.f90: the f-wave correction uses sign(s) where the wave form uses |s|). */
+int claw2d_step_aux(const claw2d_desc* d, const void* q_in_dev, void* q_out_dev, const void* aux_dev,
+                    double dt, claw2d_ctrl* ctrl_dev, void* stream);
+int claw2d_advance_aux(const claw2d_desc* d, void* q0_dev, void* q1_dev, const void* aux_dev,
+                       claw2d_ctrl* ctrl_dev, long long attempts, void* stream);
+int claw2d_step_ds_aux(const claw2d_desc* d, const void* q_in_dev, void* q_out_dev,
+                       const void* aux_dev, double dt, claw2d_ctrl* ctrl_dev, void* stream);
+int claw2d_advance_ds_aux(const claw2d_desc* d, void* q0_dev, void* q1_dev, const void* aux_dev,
+                          claw2d_ctrl* ctrl_dev, long long attempts, void* stream);
 
 /* Bytes of the workspace claw2d_mass_sum needs. */
 size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc* d);

@@ -95,3 +95,49 @@ def test_adjoint_workspace_and_validation():
         assert lib.claw2d_adjoint_step(C.byref(bad), None, None, None, None, 0.1, None) < 0
     # null buffers and a non-positive dt are refused before any launch
     assert lib.claw2d_adjoint_step(C.byref(d), None, None, None, None, 0.1, None) < 0
+
+
+def _bathy(**kw):
+    args = dict(solver_id=2, num_aux=1, fwave=1, dry_tolerance=1e-3)
+    args.update(kw)
+    return _desc(**args)
+
+
+@pytest.mark.parametrize("bad,code", [
+    (dict(num_aux=0), -16), (dict(fwave=0), -17), (dict(dry_tolerance=-1.0), -18),
+    (dict(aux_bc_lower=(0, 1)), -19), (dict(aux_bc_lower=(2, 1)), -11),
+])
+def test_validate_rejects_bathymetry(bad, code):
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    d = _bathy(**bad)
+    assert lib.claw2d_validate(C.byref(d)) == code
+    assert lib.claw2d_last_error()
+
+
+def test_aux_and_fwave_only_with_the_bathymetry_solver():
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    assert lib.claw2d_validate(C.byref(_bathy())) == 0
+    assert lib.claw2d_validate(C.byref(_desc(num_aux=1))) == -16
+    assert lib.claw2d_validate(C.byref(_desc(fwave=1))) == -17
+    assert lib.claw2d_state_bytes(C.byref(_bathy(dtype=1))) == 3 * 32 * 16 * 4
+
+
+def test_entry_points_without_aux_refuse_aux_solvers():
+    """Refused before any device call, so this runs without a GPU (the pointers are never used)."""
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    d = _bathy()
+    qa, qb, ctrl, aux = C.c_void_p(4096), C.c_void_p(8192), C.c_void_p(12288), C.c_void_p(16384)
+    assert lib.claw2d_step(C.byref(d), qa, qb, 0.1, ctrl, None) == -26
+    assert b"claw2d_step_aux" in lib.claw2d_last_error()
+    assert lib.claw2d_advance(C.byref(d), qa, qb, ctrl, 1, None) == -26
+    assert lib.claw2d_step_ds(C.byref(d), qa, qb, 0.1, ctrl, None) == -26
+    assert lib.claw2d_advance_ds(C.byref(d), qa, qb, ctrl, 1, None) == -26
+    assert lib.claw2d_step_aux(C.byref(d), qa, qb, None, 0.1, ctrl, None) == -22
+    assert lib.claw2d_advance_ds_aux(C.byref(d), qa, qb, None, ctrl, 1, None) == -22
+    assert lib.claw2d_step_aux(C.byref(d), qa, qa, aux, 0.1, ctrl, None) == -23
+    assert lib.claw2d_step_ds_aux(C.byref(d), qa, qb, aux, -0.1, ctrl, None) == -21
+    # batched grids and the adjoint take no aux solvers
+    assert lib.claw2d_adjoint_workspace_bytes(C.byref(d)) == 0

@@ -48,7 +48,7 @@ def test_defaults_follow_pyclaw():
 @pytest.mark.parametrize("attr,value,exc", [
     ("kernel_language", "Fortran", NotImplementedError),
     ("kernel_language", "Python", NotImplementedError),
-    ("fwave", True, NotImplementedError),
+    ("fwave", True, ValueError),  # shallow_roe_with_efix returns waves, not f-waves
     ("order", 3, ValueError),
     ("precision", "fp16", ValueError),
 ])
@@ -91,3 +91,49 @@ def test_no_cpu_fallback_without_device():
     solver, state, domain, sol = make()
     with pytest.raises(RuntimeError, match="HIP device"):
         solver.evolve_to_time(sol, 0.1)
+
+
+def make_bathy(nx=20, ny=10):
+    solver = pyclaw.ClawSolver2D(riemann.shallow_bathymetry_fwave_2D)
+    solver.fwave = True
+    solver.bc_lower = [pyclaw.BC.wall, pyclaw.BC.extrap]
+    solver.bc_upper = [pyclaw.BC.wall, pyclaw.BC.extrap]
+    solver.aux_bc_lower = [pyclaw.BC.wall, pyclaw.BC.extrap]
+    solver.aux_bc_upper = [pyclaw.BC.wall, pyclaw.BC.extrap]
+    domain = pyclaw.Domain((0.0, 0.0), (1.0, 1.0), (nx, ny))
+    state = pyclaw.State(domain, solver.num_eqn, 1)
+    state.problem_data.update(grav=9.81, dry_tolerance=1e-3, sea_level=0.0)
+    state.aux[0] = -1.0
+    state.q[0] = 1.0
+    return solver, state, domain, pyclaw.Solution(state, domain)
+
+
+def test_bathymetry_plugin_metadata():
+    rp = riemann.shallow_bathymetry_fwave_2D
+    assert (rp.num_eqn, rp.num_waves, rp.num_aux, rp.fwave) == (3, 3, 1, True)
+    assert riemann.static.num_aux["shallow_bathymetry_fwave_2D"] == 1
+    s = pyclaw.ClawSolver2D(rp)
+    assert (s.num_eqn, s.num_waves) == (3, 3)
+
+
+@pytest.mark.parametrize("change,exc,match", [
+    (lambda s, st: setattr(s, "fwave", False), ValueError, "fwave"),
+    (lambda s, st: setattr(st, "aux", None), ValueError, "num_aux"),
+    (lambda s, st: s.aux_bc_lower.__setitem__(0, None), Exception, "aux boundary"),
+    (lambda s, st: s.aux_bc_upper.__setitem__(1, pyclaw.BC.custom), NotImplementedError, "custom"),
+    (lambda s, st: s.aux_bc_lower.__setitem__(1, pyclaw.BC.periodic), Exception, "Periodic aux"),
+    (lambda s, st: st.problem_data.pop("dry_tolerance"), KeyError, "dry_tolerance"),
+    (lambda s, st: setattr(st, "index_capa", 0), NotImplementedError, "capacity"),
+])
+def test_bathymetry_configuration_checks(change, exc, match):
+    solver, state, domain, sol = make_bathy()
+    change(solver, state)
+    with pytest.raises(exc, match=match):
+        solver.setup(sol)
+
+
+def test_aux_with_a_solver_that_reads_none_is_refused():
+    solver, state, domain, sol = make()
+    state.aux = np.zeros((1, 20, 10), order="F")
+    with pytest.raises(NotImplementedError, match="aux"):
+        solver.setup(sol)
