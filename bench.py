@@ -421,7 +421,7 @@ def main():
                    "attempts": r64["attempts"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(a.n),
-                     "kernel": "step2_fused<double,SweP<double>,4> (one launch per attempt)",
+                     "kernel": "step2_fused<double,SweP<double>,4,false,2> (one launch per attempt)",
                      "bytes_per_cell": bytes_per_cell},
     }
     if not a.no_fp32:

@@ -141,7 +141,9 @@ __device__ __forceinline__ typename P::Cell prep_cell(const T q[P::NEQ], T b, co
 // correction carries sign(s_p) where the wave form has |s_p| (flux2.f90 with use_fwave).
 // alL/mL and alR/mR are the wave strengths of the interfaces below/above along the sweep and
 // the eigenvector products e_p(neighbour).e_p(this) the limiter needs.
-template <class T, class P, int LIM>
+// TWC >= 0: the method is known at compile time (order 2, transverse_waves = TWC), so the
+// order/transverse selects compile away; TWC < 0: order and transverse_waves from a.
+template <class T, class P, int LIM, int TWC>
 __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T alL[P::NW],
                                              const T mL[P::NW], const T alR[P::NW],
                                              const T mR[P::NW], T r, T rt,
@@ -149,8 +151,9 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
                                              T LG2[P::NEQ], T RX[P::NEQ], T RG1[P::NEQ],
                                              T RG2[P::NEQ]) {
   constexpr int NEQ = P::NEQ, NW = P::NW;
-  const bool second = a.order != 1;
-  const bool fold = second && a.tw == 2;
+  const bool second = TWC >= 0 ? true : a.order != 1;
+  const int tw = TWC >= 0 ? TWC : a.tw;
+  const bool fold = second && tw == 2;
   T in[NW];
   if (second) P::inv_norm(f, in);
   T XL[NW], XR[NW], A[NW], B[NW];
@@ -176,7 +179,7 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
   P::expand_n(f, XR, RX);
 #pragma unroll
   for (int m = 0; m < NEQ; ++m) LG1[m] = LG2[m] = RG1[m] = RG2[m] = T(0);
-  if (a.tw > 0) {
+  if (tw > 0) {
     T bA[NW], bB[NW];
     P::tbeta(f, A, bA);
     P::tbeta(f, B, bB);
@@ -217,7 +220,7 @@ template <int NEQ, class T> __device__ __forceinline__ void yswap(T v[NEQ]) {
 // from qin to qout.  Shared by the single-grid and the batched kernel.  The CFL speeds and the
 // bad-state flag of the item are reduced over the wavefront and folded into *ctrl atomically.
 // LIM > 0: every wave uses limiter LIM (compile-time); LIM == 0: per-wave ids from a.mthlim.
-template <class T, class P, int LIM, bool SPLIT = false>
+template <class T, class P, int LIM, bool SPLIT = false, int TWC = -1>
 __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __restrict__ qin,
                                            T* __restrict__ qout, const int mx, const int my,
                                            const int x0, const int y0, const int y1, const double dt,
@@ -336,7 +339,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         P::mdot(lp, ln, mR);
         if (R >= y0 + 1) {
           T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-          finish_iface<T, P, LIM>(fyprev, yal2, ymprev, fynew.al, mR, dtdy, dtdx, a, LX, LG1, LG2,
+          finish_iface<T, P, LIM, TWC>(fyprev, yal2, ymprev, fynew.al, mR, dtdy, dtdx, a, LX, LG1, LG2,
                                   RX, RG1, RG2);
           yswap<NEQ>(LX); yswap<NEQ>(LG1); yswap<NEQ>(LG2);
           yswap<NEQ>(RX); yswap<NEQ>(RG1); yswap<NEQ>(RG2);
@@ -420,7 +423,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         alL[p] = lleft[p];
       }
       T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-      finish_iface<T, P, LIM>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2);
+      finish_iface<T, P, LIM, TWC>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2);
       if constexpr (SPLIT) {
         // q* of row R: the x-sweep only (no transverse terms); valid on the owned columns
         T qs[NEQ];
@@ -435,7 +438,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
           const T G2 = RG2[m] + shfl(LG2[m], addrR);
           acc1[m] -= G1;
           acc2[m] += (X + G1) - G2;
-          acc3[m] += G2;
+          acc3[m] = G2;  // row R+1 has no other contribution yet
         }
       }
     }
@@ -561,7 +564,7 @@ template <class T> struct MinWaves {
   static constexpr int value = sizeof(T) == 8 ? CLAW_MIN_WAVES_F64 : CLAW_MIN_WAVES_F32;
 };
 
-template <class T, class P, int LIM, bool SPLIT>
+template <class T, class P, int LIM, bool SPLIT, int TWC>
 __global__ void __launch_bounds__(kLanes* kWPB) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::value)))
 step2_fused(StepArgs<T, P> a) {
   double dt = a.dt_fixed;
@@ -578,7 +581,7 @@ step2_fused(StepArgs<T, P> a) {
   const int strip = wave % a.nstrips, seg = wave / a.nstrips;
   const int x0 = strip * kOwn, y0 = seg * a.seg_h;
   const int y1 = min(y0 + a.seg_h, a.my);
-  march_item<T, P, LIM, SPLIT>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
+  march_item<T, P, LIM, SPLIT, TWC>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
   if (a.fuse) {
     // The last wavefront to finish closes the step: its speed maxima are then complete.  The
     // wait for this wavefront's outstanding memory operations (its max/or atomics are acknowledged
@@ -611,7 +614,7 @@ struct BatchDev {
   int total_items;
 };
 
-template <class T, class P, int LIM>
+template <class T, class P, int LIM, int TWC>
 __global__ void __launch_bounds__(kLanes* kWPB) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::value)))
 batch_step(StepArgs<T, P> a, BatchDev b) {
   const int lane = threadIdx.x & 63;
@@ -637,7 +640,7 @@ batch_step(StepArgs<T, P> a, BatchDev b) {
     const int strip = local % G.nstrips, seg = local / G.nstrips;
     const int x0 = strip * kOwn, y0 = seg * G.seg_h;
     const int y1 = min(y0 + G.seg_h, G.my);
-    march_item<T, P, LIM>(a, qin, qout, G.mx, G.my, x0, y0, y1, dt, G.dx, G.dy, c);
+    march_item<T, P, LIM, false, TWC>(a, qin, qout, G.mx, G.my, x0, y0, y1, dt, G.dx, G.dy, c);
   }
 }
 
@@ -812,17 +815,40 @@ template <class T> static typename EulerP<T>::Par euler_par(const claw2d_desc* d
   return p;
 }
 
+// Kernel variant for a method: a uniform MC or minmod limiter at order 2 with the transverse
+// mode fixed at compile time (split: none; unsplit: 1 or 2), else the general kernel (per-wave
+// limiter ids, order and transverse_waves read at run time).  slot indexes the occupancy cache.
 template <class T, class P, bool SPLIT>
-static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q0, void* q1,
-                       const void* aux, claw2d_ctrl* ctrl, int use_ctrl, double dt, hipStream_t s) {
+static const void* step_kernel(const claw2d_desc* d, int* slot) {
   bool same = true;
   for (int p = 1; p < P::NW; ++p) same = same && d->mthlim[p] == d->mthlim[0];
   const int lim = same ? d->mthlim[0] : -1;
-  const void* kern = lim == 4 ? (const void*)step2_fused<T, P, 4, SPLIT>
-                   : (lim == 1 ? (const void*)step2_fused<T, P, 1, SPLIT>
-                               : (const void*)step2_fused<T, P, 0, SPLIT>);
-  static thread_local long cached[3] = {-1, -1, -1};
-  const int slot = lim == 4 ? 0 : (lim == 1 ? 1 : 2);
+  if ((lim == 4 || lim == 1) && d->order == 2) {
+    const bool mc = lim == 4;
+    if constexpr (SPLIT) {
+      *slot = mc ? 0 : 1;
+      return mc ? (const void*)step2_fused<T, P, 4, true, 0> : (const void*)step2_fused<T, P, 1, true, 0>;
+    } else {
+      if (d->transverse_waves == 2) {
+        *slot = mc ? 0 : 1;
+        return mc ? (const void*)step2_fused<T, P, 4, false, 2> : (const void*)step2_fused<T, P, 1, false, 2>;
+      }
+      if (d->transverse_waves == 1) {
+        *slot = mc ? 2 : 3;
+        return mc ? (const void*)step2_fused<T, P, 4, false, 1> : (const void*)step2_fused<T, P, 1, false, 1>;
+      }
+    }
+  }
+  *slot = 4;
+  return (const void*)step2_fused<T, P, 0, SPLIT, -1>;
+}
+
+template <class T, class P, bool SPLIT>
+static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q0, void* q1,
+                       const void* aux, claw2d_ctrl* ctrl, int use_ctrl, double dt, hipStream_t s) {
+  int slot = 0;
+  const void* kern = step_kernel<T, P, SPLIT>(d, &slot);
+  static thread_local long cached[5] = {-1, -1, -1, -1, -1};
   if (cached[slot] < 0) cached[slot] = resident_waves(kern);
   StepArgs<T, P> a;
   fill_args(a, d, cached[slot]);
@@ -841,13 +867,8 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   const int waves = a.nstrips * a.nsegs;
   a.total_waves = waves;
   const int blocks = (waves + kWPB - 1) / kWPB;
-  if (lim == 4)
-    hipLaunchKernelGGL((step2_fused<T, P, 4, SPLIT>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
-  else if (lim == 1)
-    hipLaunchKernelGGL((step2_fused<T, P, 1, SPLIT>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
-  else
-    hipLaunchKernelGGL((step2_fused<T, P, 0, SPLIT>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a);
-  const hipError_t e = hipGetLastError();
+  void* args[] = {&a};
+  const hipError_t e = hipLaunchKernel(kern, dim3(blocks), dim3(kLanes * kWPB), args, 0, s);
   return e == hipSuccess ? 0 : hip_fail(e, "step2_fused launch");
 }
 
@@ -871,17 +892,30 @@ static int dispatch_step(const claw2d_desc* d, void* q0, void* q1, const void* a
                                : dispatch_typed<float, SPLIT>(d, q0, q1, aux, ctrl, use_ctrl, dt, s);
 }
 
+// The batched kernel for a method, chosen as step_kernel chooses the single-grid one, so that a
+// grid advanced in a batch and alone runs the same arithmetic.
+template <class T, class P>
+static const void* batch_kernel(const claw2d_desc* d, int* slot) {
+  bool same = true;
+  for (int p = 1; p < P::NW; ++p) same = same && d->mthlim[p] == d->mthlim[0];
+  const int lim = same ? d->mthlim[0] : -1;
+  if ((lim == 4 || lim == 1) && d->order == 2 && (d->transverse_waves == 1 || d->transverse_waves == 2)) {
+    const bool mc = lim == 4, t2 = d->transverse_waves == 2;
+    *slot = (mc ? 0 : 1) + (t2 ? 0 : 2);
+    if (mc) return t2 ? (const void*)batch_step<T, P, 4, 2> : (const void*)batch_step<T, P, 4, 1>;
+    return t2 ? (const void*)batch_step<T, P, 1, 2> : (const void*)batch_step<T, P, 1, 1>;
+  }
+  *slot = 4;
+  return (const void*)batch_step<T, P, 0, -1>;
+}
+
 template <class T, class P>
 static int launch_batch(const claw2d_desc* d, const typename P::Par& par, const claw2d_batch_grid* grids,
                         int ngrids, int total_items, void* q0, void* q1, claw2d_ctrl* ctrls,
                         int32_t* counter, hipStream_t s) {
-  bool same = true;
-  for (int p = 1; p < P::NW; ++p) same = same && d->mthlim[p] == d->mthlim[0];
-  const int lim = same ? d->mthlim[0] : -1;
-  const void* kern = lim == 4 ? (const void*)batch_step<T, P, 4>
-                   : (lim == 1 ? (const void*)batch_step<T, P, 1> : (const void*)batch_step<T, P, 0>);
-  static thread_local long cached[3] = {-1, -1, -1};
-  const int slot = lim == 4 ? 0 : (lim == 1 ? 1 : 2);
+  int slot = 0;
+  const void* kern = batch_kernel<T, P>(d, &slot);
+  static thread_local long cached[5] = {-1, -1, -1, -1, -1};
   if (cached[slot] < 0) cached[slot] = resident_waves(kern);
   StepArgs<T, P> a;
   fill_args(a, d, cached[slot]);
@@ -903,13 +937,8 @@ static int launch_batch(const claw2d_desc* d, const typename P::Par& par, const 
   const long res = cached[slot] > 0 ? cached[slot] : 4096;
   const long waves = std::min<long>(total_items, res);
   const int blocks = (int)((waves + kWPB - 1) / kWPB);
-  if (lim == 4)
-    hipLaunchKernelGGL((batch_step<T, P, 4>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a, b);
-  else if (lim == 1)
-    hipLaunchKernelGGL((batch_step<T, P, 1>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a, b);
-  else
-    hipLaunchKernelGGL((batch_step<T, P, 0>), dim3(blocks), dim3(kLanes * kWPB), 0, s, a, b);
-  const hipError_t e = hipGetLastError();
+  void* args[] = {&a, &b};
+  const hipError_t e = hipLaunchKernel(kern, dim3(blocks), dim3(kLanes * kWPB), args, 0, s);
   return e == hipSuccess ? 0 : hip_fail(e, "batch_step launch");
 }
 

@@ -64,20 +64,22 @@ __device__ __forceinline__ double vmin(double a, double b) { return __builtin_fm
 __device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
 __device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double vabs(double a) { return __builtin_fabs(a); }
+__device__ __forceinline__ float vabs(float a) { return __builtin_fabsf(a); }
 
 // phi(t/alpha)*alpha for limiter `meth` (ids of pyclaw/limiters/tvd.py: minmod 1, superbee 2,
 // van Leer 3, MC 4, beam-warming 5, 0 = unlimited).  alpha == 0 gives 0 (upstream skips a wave
-// of zero norm, which is the zero wave).
+// of zero norm, which is the zero wave).  minmod, superbee and MC are odd in (alpha, t), so they
+// are evaluated on |alpha| and sign(alpha) t and the sign is put back: the same values, bit for bit,
+// as the two-branch form (negation and the factors 2 and 1/2 are exact), with no selects.
 template <class T> __device__ __forceinline__ T limited(T al, T t, int meth) {
+  const T sg = copysign(T(1), al), a = vabs(al), ts = sg * t;
   switch (meth) {
-    case 1: {  // minmod: max(0, min(1, theta))
-      const T lo = vmin(al, t), hi = vmax(al, t);
-      return al > T(0) ? vmax(T(0), lo) : vmin(T(0), hi);
-    }
+    case 1:  // minmod: max(0, min(1, theta))
+      return sg * vmax(T(0), vmin(a, ts));
     case 2: {  // superbee: max(0, min(1, 2 theta), min(2, theta))
-      const T t2 = T(2) * t, a2 = T(2) * al;
-      const T lo = vmax(vmin(al, t2), vmin(a2, t)), hi = vmin(vmax(al, t2), vmax(a2, t));
-      return al > T(0) ? vmax(T(0), lo) : vmin(T(0), hi);
+      const T lo = vmax(vmin(a, T(2) * ts), vmin(T(2) * a, ts));
+      return sg * vmax(T(0), lo);
     }
     case 3: {  // van Leer: (theta + |theta|)/(1 + |theta|)
       if (al == T(0)) return T(0);
@@ -85,9 +87,8 @@ template <class T> __device__ __forceinline__ T limited(T al, T t, int meth) {
       return al * ((th + ath) / (T(1) + ath));
     }
     case 4: {  // MC: max(0, min((1 + theta)/2, 2, 2 theta))
-      const T c = T(0.5) * (al + t), t2 = T(2) * t, a2 = T(2) * al;
-      const T lo = vmin(vmin(c, t2), a2), hi = vmax(vmax(c, t2), a2);
-      return al > T(0) ? vmax(T(0), lo) : vmin(T(0), hi);
+      const T c = T(0.5) * (a + ts);
+      return sg * vmax(T(0), vmin(c, T(2) * vmin(a, ts)));
     }
     case 5: return al == T(0) ? T(0) : t;  // beam-warming: theta
     default: return al;
