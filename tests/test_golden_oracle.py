@@ -17,12 +17,15 @@ def test_oracle_reproduces_fixture(path):
     spec.loader.exec_module(mk)
     name = os.path.basename(path)[:-4]
     ref = np.load(path)
-    q0, qn, st, dx, dy = mk.run_case(name)
+    q0, qn, st, dx, dy, aux = mk.run_case(name)
     assert np.array_equal(q0, ref["q0"])
+    if aux is not None:
+        assert np.array_equal(aux, ref["aux"])
     assert np.array_equal(qn, ref["q"])
     assert st["t"] == float(ref["t"]) and st["numsteps"] == int(ref["numsteps"])
     assert st["rejected"] == int(ref["rejected"]) and st["dt"] == float(ref["dt"])
 
 
 def test_fixture_set_present():
-    assert len(FIXTURES) >= 4
+    assert len(FIXTURES) >= 6
+    assert any("bathy" in p for p in FIXTURES)

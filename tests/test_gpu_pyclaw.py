@@ -66,11 +66,17 @@ def test_golden_fixture_on_device(path):
     solver = str(ref["solver"])
     q0 = ref["q0"]
     neq, mx, my = q0.shape
-    bc = tuple(int(b) for b in ref["bc"])
-    d = make_desc(0 if solver == "swe" else 1, 0, mx, my, float(ref["dx"]), float(ref["dy"]), neq, neq,
-                  bc_lower=bc, bc_upper=bc, grav=1.0, gamma=1.4)
-    g = DeviceGrid(d)
+    bc = np.array(ref["bc"])
+    lo, hi = (tuple(bc[0]), tuple(bc[1])) if bc.ndim == 2 else (tuple(bc), tuple(bc))
+    bathy = solver == "bathy"
+    sid = {"swe": 0, "euler": 1, "bathy": 2}[solver]
+    d = make_desc(sid, 0, mx, my, float(ref["dx"]), float(ref["dy"]), neq, neq,
+                  bc_lower=lo, bc_upper=hi, grav=float(ref["grav"]) if bathy else 1.0, gamma=1.4,
+                  dry_tolerance=float(ref["dry_tolerance"]) if bathy else 1e-3)
+    g = DeviceGrid(d, split=bool(ref["split"]) if bathy else False)
     g.upload(q0)
+    if bathy:
+        g.upload_aux(ref["aux"])
     steps = int(ref["numsteps"])
     g.init_ctrl(0.0, 0.1, target_steps=steps)
     g.advance(steps + int(ref["rejected"]) + 2)
