@@ -158,6 +158,19 @@ class DeviceGrid:
         c = self.read_ctrl()
         return c.cfl, c
 
+    def step_into(self, src, dst, dt, stream=None):
+        """One fixed-dt step from the planes `src` into the planes `dst` (device tensors of this
+        grid's shape and dtype; they may be any buffers, cur is not changed).  The CFL number and
+        status bits land in the control block, as for step()."""
+        for t in (src, dst):
+            if tuple(t.shape) != tuple(self.q[0].shape) or t.dtype != self.torch_dtype or not t.is_contiguous():
+                raise ValueError("step_into needs contiguous planes of shape %s" % (tuple(self.q[0].shape),))
+        if src.data_ptr() == dst.data_ptr():
+            raise ValueError("step_into: src and dst must differ")
+        check(self._step_fn()(C.byref(self.desc), C.c_void_p(src.data_ptr()),
+                              C.c_void_p(dst.data_ptr()), float(dt), self.ctrl_ptr,
+                              _stream_handle(stream)), "claw2d_step")
+
     def enqueue_step(self, dt, stream=None):
         """Like step() but without reading the control block back (no host synchronisation)."""
         src, dst = self.q[self.cur], self.q[1 - self.cur]

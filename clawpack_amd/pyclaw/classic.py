@@ -2,12 +2,16 @@
 
 The unsplit classic step (step2/flux2/limiter with rpn2/rpt2) and the evolve_to_time loop run on
 the GPU through libclawpack_amd.so: one fused kernel per step attempt plus a device-side
-accept/reject/dt controller, so the host only synchronises at output times.  There is no CPU
-compute path: a missing library or device raises.
+accept/reject/dt controller, so the host only synchronises at output times.  With a step_source
+or before_step callback (upstream Solver.step: Godunov or Strang source splitting) the loop runs
+on the host, one step launch per attempt, and the callbacks act on the device state through torch
+(DeviceState).  There is no CPU compute path: a missing library or device raises.
 """
 import math
 
 import numpy as np
+
+import torch
 
 from .. import _lib
 from ..device import DeviceGrid, make_desc
@@ -15,6 +19,36 @@ from .limiters import tvd
 from .solver import BC, Solver
 
 _KERNEL_LANGUAGES = ("HIP",)
+
+
+class DeviceState:
+    """The state that step_source(solver, state, dt) and before_step(solver, state) receive on the
+    HIP path.  q (num_eqn, mx, my) and aux (num_aux, mx, my) are torch device tensors in pyclaw's
+    layout, views of the device planes: in-place updates (state.q[1] *= ..., state.q[1:3].copy_(...))
+    change the device state, with no host copy.  problem_data, grid, patch and t are the host
+    State's."""
+
+    def __init__(self, state, grid):
+        self.problem_data = state.problem_data
+        self.grid = state.grid
+        self.patch = state.patch
+        self.index_capa = state.index_capa
+        self.aux = None if grid.aux is None else grid.aux.permute(0, 2, 1)
+        self.q = None
+        self.t = state.t
+
+    def bind(self, planes, t):
+        self.q = planes.permute(0, 2, 1)
+        self.t = t
+        return self
+
+    @property
+    def num_eqn(self):
+        return self.q.shape[0]
+
+    @property
+    def num_aux(self):
+        return 0 if self.aux is None else self.aux.shape[0]
 
 
 class ClawSolver2D(Solver):
@@ -85,9 +119,8 @@ class ClawSolver2D(Solver):
                     raise Exception("Periodic aux boundary conditions must be set on both sides")
         if self.num_ghost != 2:
             raise NotImplementedError("num_ghost must be 2")
-        if self.before_step is not None or self.step_source is not None:
-            raise NotImplementedError("before_step/step_source callbacks are not supported on the HIP "
-                                      "path (the time loop runs on the device)")
+        if self.step_source is not None and self.source_split not in (1, 2):
+            raise ValueError("source_split must be 1 (Godunov) or 2 (Strang)")
         if self.order not in (1, 2) or self.transverse_waves not in (0, 1, 2):
             raise ValueError("order must be 1 or 2 and transverse_waves 0, 1 or 2")
         for k in range(2):
@@ -174,6 +207,8 @@ class ClawSolver2D(Solver):
         state = solution.state
         self._check_supported(state)
         self._check_cfl_settings()
+        if self.step_source is not None or self.before_step is not None:
+            return self._evolve_with_callbacks(solution, tend)
         g = self._device(state)
         tstart = solution.t
         take_one_step = tend is None
@@ -213,6 +248,92 @@ class ClawSolver2D(Solver):
             raise Exception("CFL too large, giving up!")
         self.dt = c.dt
         if (c.status & _lib.STATUS_MAX_STEPS) or (not take_one_step and not c.done):
+            raise Exception("Maximum number of timesteps have been taken")
+        return self.status
+
+    def _evolve_with_callbacks(self, solution, tend):
+        """evolve_to_time with before_step / step_source (upstream Solver.step and
+        evolve_to_time): per attempt before_step, for Strang splitting a half source step, the
+        hyperbolic step, and after an accepted step the source over dt (Godunov) or dt/2
+        (Strang).  The accept/reject/dt rules are those of the device controller, applied on the
+        host after each launch.  A rejected attempt leaves q^n untouched (the Strang half step
+        works on a copy)."""
+        state = solution.state
+        g = self._device(state)
+        g.upload(state.q)
+        ds = DeviceState(state, g)
+        take_one_step = tend is None
+        strang = self.step_source is not None and self.source_split == 2
+        tmp = torch.empty_like(g.q[0]) if strang else None
+        t = tstart = solution.t
+        dt = self.dt
+        if not take_one_step:
+            if t + dt > tend and tstart < tend:
+                dt = tend - t
+            if tend - t - dt < 1.0e-14 * t:
+                dt = tend - t
+        g.init_ctrl(t, max(dt, 1e-300))  # zeroes the speed accumulators and status bits
+        cflmax = self.cfl.get_cached_max()
+        dtmin = dtmax = dt
+        numsteps = rejected = 0
+        done = False
+        cfl = 0.0
+        for attempt in range(1 if take_one_step else self.max_steps):
+            if self.before_step is not None:
+                self.before_step(self, ds.bind(g.q[g.cur], t))
+            src = g.q[g.cur]
+            if strang:
+                tmp.copy_(src)
+                self.step_source(self, ds.bind(tmp, t), dt / 2.0)
+                src = tmp
+            g.step_into(src, g.q[1 - g.cur], dt)
+            c = g.read_ctrl()
+            cfl = c.cfl
+            bad = bool(c.status & _lib.STATUS_BAD_STATE) or cfl != cfl
+            if bad and not (cfl > self.cfl_max):
+                solution.t = t
+                raise FloatingPointError("HIP step produced a non-positive depth/density or a NaN "
+                                         "wave speed at t=%g" % t)
+            if c.status:
+                c.status = 0
+                g.write_ctrl(c)
+            if not (cfl > self.cfl_max):
+                cflmax = max(cfl, cflmax)
+                numsteps += 1
+                t = t + dt if self.dt_variable else tstart + numsteps * dt
+                g.cur = 1 - g.cur
+                if self.step_source is not None:
+                    self.step_source(self, ds.bind(g.q[g.cur], t), dt / 2.0 if strang else dt)
+            else:
+                rejected += 1
+                if not self.dt_variable:
+                    cflmax = max(cfl, cflmax)
+                    self.cfl.update_global_max(cfl)
+                    raise Exception("CFL too large, giving up!")
+            if take_one_step or t >= tend:
+                done = True
+                break
+            if self.dt_variable:
+                if cfl > 0.0:
+                    dt = min(self.dt_max, dt * self.cfl_desired / cfl)
+                    dtmin = min(dt, dtmin)
+                    dtmax = max(dt, dtmax)
+                else:
+                    dt = self.dt_max
+            if t + dt > tend and tstart < tend:
+                dt = tend - t
+            if tend - t - dt < 1.0e-14 * t:
+                dt = tend - t
+        state.q[...] = g.download()
+        solution.t = t
+        self.cfl.update_global_max(cfl)
+        self.status["cflmax"] = cflmax
+        self.status["dtmin"] = dtmin
+        self.status["dtmax"] = dtmax
+        self.status["numsteps"] = numsteps
+        self.status["rejected"] = rejected
+        self.dt = dt  # the dt of the last attempt (pyclaw leaves the loop before choosing a new one)
+        if not done:
             raise Exception("Maximum number of timesteps have been taken")
         return self.status
 

@@ -283,3 +283,59 @@ def philim(a, b, meth):
 def component_sum(d, q, m):
     q = np.asfortranarray(q, dtype=np.float64)
     return lib().orc_component_sum(C.byref(d), _p(q), m)
+
+
+def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, nsteps=None,
+                       tend=None, t0=0.0, dt=None, aux=None):
+    """Solver.evolve_to_time with Solver.step's source splitting (upstream pyclaw/src/pyclaw/
+    solver.py, absent from the reference tree): per attempt before_step(q, t), for Strang
+    splitting (source_split 2) a half step source(q, dt/2) before the hyperbolic step, and after
+    an accepted step source(q, dt) (Godunov, source_split 1) or source(q, dt/2) (Strang); a
+    rejected attempt restores q and t.  source and before_step act on q (num_eqn, mx, my) in
+    place.  The accept/reject, t and dt rules are orc_evolve's; nsteps stops after that many
+    accepted steps (no clipping to tend), tend runs to that time.  The hyperbolic step is
+    orc_step_hyperbolic."""
+    q = np.array(q, dtype=np.float64, order="F", copy=True)
+    t = tstart = t0
+    dt = c.dt_initial if dt is None else dt
+    st = dict(numsteps=0, rejected=0, attempts=0, cflmax=0.0, dtmin=dt, dtmax=dt, error=0)
+    to_tend = nsteps is None
+    for _ in range(c.max_steps):
+        if to_tend:
+            if t + dt > tend and tstart < tend:
+                dt = tend - t
+            if tend - t - dt < 1.0e-14 * t:
+                dt = tend - t
+        qbackup = q.copy(order="F")
+        if before_step is not None:
+            before_step(q, t)
+        if source is not None and source_split == 2:
+            source(q, dt / 2.0)
+        qn, cfl = step_hyperbolic(d, c, q, dt, aux=aux)
+        st["attempts"] += 1
+        if not (cfl > c.cfl_max):
+            q = qn
+            st["cflmax"] = max(cfl, st["cflmax"])
+            st["numsteps"] += 1
+            t = t + dt if c.dt_variable else tstart + st["numsteps"] * dt
+            if source is not None:
+                source(q, dt / 2.0 if source_split == 2 else dt)
+        else:
+            st["rejected"] += 1
+            q = qbackup
+            if not c.dt_variable:
+                st["error"] = 2
+                break
+        if (st["numsteps"] >= nsteps) if not to_tend else (t >= tend):
+            break
+        if c.dt_variable:
+            if cfl > 0.0:
+                dt = min(c.dt_max, dt * c.cfl_desired / cfl)
+                st["dtmin"] = min(dt, st["dtmin"])
+                st["dtmax"] = max(dt, st["dtmax"])
+            else:
+                dt = c.dt_max
+    else:
+        st["error"] = 1
+    st["t"], st["dt"] = t, dt
+    return q, st

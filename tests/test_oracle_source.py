@@ -1,0 +1,78 @@
+"""Pins of the oracle's source-splitting loop (corc.evolve_with_source: Solver.evolve_to_time with
+Solver.step's Godunov / Strang splitting of a step_source callback).
+
+  * with no source it is orc_evolve bit for bit (to tend and for a step count);
+  * a Coriolis source on a uniform periodic flow: the hyperbolic step leaves a uniform state
+    unchanged, so the velocity must rotate by exactly f t (closed form), for both splittings;
+  * Strang splitting applies half source steps on either side: with a damping source, k dt/2
+    before and after each step, so a uniform state decays by exp(-k t) like Godunov's;
+  * before_step runs once per attempt, rejected attempts included, and a rejected attempt
+    leaves no trace of its Strang half step.
+"""
+import numpy as np
+import pytest
+
+from clawpack_amd import problems
+from oracle import corc
+
+
+def coriolis(f):
+    def src(q, dt):
+        c, s = np.cos(f * dt), np.sin(f * dt)
+        hu = q[1].copy()
+        q[1] = c * hu + s * q[2]
+        q[2] = -s * hu + c * q[2]
+    return src
+
+
+def test_no_source_is_orc_evolve():
+    q, dx, dy = problems.radial_dam_break(40, 36)
+    d = corc.make_desc("swe", 40, 36, dx, dy, grav=1.0)
+    for split in (False, True):
+        c = corc.make_ctrl(dimensional_split=split)
+        a, sa = corc.evolve(d, c, q, tend=0.35)
+        b, sb = corc.evolve_with_source(d, c, q, tend=0.35)
+        assert np.array_equal(a, b)
+        assert (sa["numsteps"], sa["rejected"], sa["t"], sa["dt"]) == (sb["numsteps"], sb["rejected"], sb["t"], sb["dt"])
+
+
+@pytest.mark.parametrize("split_src", [1, 2])
+def test_coriolis_rotates_a_uniform_flow_exactly(split_src):
+    mx, my, f = 24, 20, 3.0
+    q = np.zeros((3, mx, my), order="F")
+    q[0], q[1], q[2] = 1.5, 0.3, -0.2
+    d = corc.make_desc("swe", mx, my, 1.0 / mx, 1.0 / my, grav=1.0)
+    c = corc.make_ctrl(bc_lower=(2, 2), bc_upper=(2, 2), dt_initial=0.01)
+    qn, st = corc.evolve_with_source(d, c, q, source=coriolis(f), source_split=split_src, tend=1.3)
+    assert abs(st["t"] - 1.3) < 1e-14 and st["numsteps"] > 20
+    th = f * st["t"]
+    hu = np.cos(th) * 0.3 + np.sin(th) * (-0.2)
+    hv = -np.sin(th) * 0.3 + np.cos(th) * (-0.2)
+    assert np.abs(qn[0] - 1.5).max() == 0.0
+    assert np.abs(qn[1] - hu).max() <= 1e-13 and np.abs(qn[2] - hv).max() <= 1e-13
+
+
+def test_strang_half_steps_and_rejected_attempts():
+    mx, my, k = 30, 30, 2.0
+    q, dx, dy = problems.radial_dam_break(mx, my)
+    q[1] = 0.2
+    d = corc.make_desc("swe", mx, my, dx, dy, grav=1.0)
+    c = corc.make_ctrl(bc_lower=(2, 2), bc_upper=(2, 2), dt_initial=0.5)  # the first attempt is rejected
+    log = []
+
+    def damp(q, dt):
+        log.append(("src", dt))
+        q[1:] *= np.exp(-k * dt)
+
+    def before(q, t):
+        log.append(("before", t))
+
+    qn, st = corc.evolve_with_source(d, c, q, source=damp, source_split=2, before_step=before, nsteps=6)
+    assert st["rejected"] >= 1 and st["numsteps"] == 6
+    assert sum(1 for e in log if e[0] == "before") == st["attempts"]
+    # two half steps per accepted attempt and one per rejected attempt
+    assert sum(1 for e in log if e[0] == "src") == 2 * st["numsteps"] + st["rejected"]
+    # total momentum decays exactly like exp(-k t) (the hyperbolic step conserves it, periodic)
+    m0 = corc.component_sum(d, q, 1)
+    m1 = corc.component_sum(d, qn, 1)
+    assert abs(m1 - m0 * np.exp(-k * st["t"])) <= 1e-12 * abs(m0)
