@@ -1,11 +1,13 @@
 """Throughput of the fused 2D unsplit step on one MI355X.
 
-Workload (north-star headline): 2D shallow water, Roe + entropy fix, unsplit, transverse_waves=2,
+Headline workload: 2D shallow water, Roe + entropy fix, unsplit, transverse_waves=2,
 MC limiter, CFL 0.9, periodic BCs, 4096x4096 cells on [-2.5,2.5]^2, initial h = 1 + 16 seeded
 Gaussian humps, hu = hv = 0.  One "step" = one accepted time step of the whole grid (fused step
 kernel + device-side CFL/dt controller), state resident in HBM.
 
-Prints one JSON line: cell-updates/s (FP64), roofline of the step kernel, FP32 rate, CPU baseline.
+Prints one JSON line: cell-updates/s (FP64), roofline of the step kernel, and the other lines
+(FP32, split step, batched grids, Euler, bathymetry, FP32-vs-FP64 error, adjoint cost, CPU
+baseline), each of which can be switched off with its --no-* flag.
 """
 import argparse
 import json
