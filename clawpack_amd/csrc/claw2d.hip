@@ -220,7 +220,9 @@ template <int NEQ, class T> __device__ __forceinline__ void yswap(T v[NEQ]) {
 // from qin to qout.  Shared by the single-grid and the batched kernel.  The CFL speeds and the
 // bad-state flag of the item are reduced over the wavefront and folded into *ctrl atomically.
 // LIM > 0: every wave uses limiter LIM (compile-time); LIM == 0: per-wave ids from a.mthlim.
-template <class T, class P, int LIM, bool SPLIT = false, int TWC = -1>
+// PAD: qin is a ghost-padded state (num_eqn planes of my+4 rows of mx+4 cells, ghost cells filled
+// by the caller, as classic step2 receives qbc); q's boundary conditions are then not applied here.
+template <class T, class P, int LIM, bool SPLIT = false, int TWC = -1, bool PAD = false>
 __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __restrict__ qin,
                                            T* __restrict__ qout, const int mx, const int my,
                                            const int x0, const int y0, const int y1, const double dt,
@@ -235,8 +237,9 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   const size_t plane = (size_t)mx * (size_t)my;
   const int c = x0 - 2 + lane;
   int cx;
-  bool fx;
-  map_index(c, mx, a.bcx_lo, a.bcx_hi, cx, fx);
+  bool fx = false;
+  if constexpr (PAD) cx = c < -2 ? -2 : (c > mx + 1 ? mx + 1 : c);  // lanes past the halo: unused data
+  else map_index(c, mx, a.bcx_lo, a.bcx_hi, cx, fx);
   int cxa = cx;  // aux column (aux BCs may differ from the state's)
   if constexpr (P::NAUX > 0) {
     bool fa;
@@ -251,15 +254,20 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
 
   // Loads row R of this lane's column.  The wall sign flips are returned, not applied, so the
   // consumer (one row later) and not the load waits for the data.
-  const bool walls = fx || a.bcy_lo == CLAW_BC_WALL || a.bcy_hi == CLAW_BC_WALL;
+  const bool walls = !PAD && (fx || a.bcy_lo == CLAW_BC_WALL || a.bcy_hi == CLAW_BC_WALL);
+  const ptrdiff_t ldq = PAD ? (ptrdiff_t)mx + 4 : (ptrdiff_t)mx;  // row stride of qin
+  const size_t plane_in = PAD ? (size_t)(mx + 4) * (size_t)(my + 4) : plane;
+  const T* const qin0 = PAD ? qin + 2 * ldq + 2 : qin;             // cell (0, 0)
   auto load_row = [&](int R, T q[NEQ], bool& fy, T& b) {
     int ry = R;
     fy = false;
-    if (R < 0 || R >= my) map_index(R, my, a.bcy_lo, a.bcy_hi, ry, fy);
+    if constexpr (!PAD) {
+      if (R < 0 || R >= my) map_index(R, my, a.bcy_lo, a.bcy_hi, ry, fy);
+    }
     ry = __builtin_amdgcn_readfirstlane(ry);
-    const T* src = qin + (size_t)ry * (size_t)mx + (size_t)cx;
+    const T* src = qin0 + (ptrdiff_t)ry * ldq + (ptrdiff_t)cx;
 #pragma unroll
-    for (int m = 0; m < NEQ; ++m) q[m] = src[(size_t)m * plane];
+    for (int m = 0; m < NEQ; ++m) q[m] = src[(size_t)m * plane_in];
     if constexpr (P::NAUX > 0) {
       int ra = R;
       bool fa;
@@ -564,7 +572,7 @@ template <class T> struct MinWaves {
   static constexpr int value = sizeof(T) == 8 ? CLAW_MIN_WAVES_F64 : CLAW_MIN_WAVES_F32;
 };
 
-template <class T, class P, int LIM, bool SPLIT, int TWC>
+template <class T, class P, int LIM, bool SPLIT, int TWC, bool PAD = false>
 __global__ void __launch_bounds__(kLanes* kWPB) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::value)))
 step2_fused(StepArgs<T, P> a) {
   double dt = a.dt_fixed;
@@ -581,7 +589,7 @@ step2_fused(StepArgs<T, P> a) {
   const int strip = wave % a.nstrips, seg = wave / a.nstrips;
   const int x0 = strip * kOwn, y0 = seg * a.seg_h;
   const int y1 = min(y0 + a.seg_h, a.my);
-  march_item<T, P, LIM, SPLIT, TWC>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
+  march_item<T, P, LIM, SPLIT, TWC, PAD>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
   if (a.fuse) {
     // The last wavefront to finish closes the step: its speed maxima are then complete.  The
     // wait for this wavefront's outstanding memory operations (its max/or atomics are acknowledged
@@ -845,10 +853,12 @@ static const void* step_kernel(const claw2d_desc* d, int* slot) {
 
 template <class T, class P, bool SPLIT>
 static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q0, void* q1,
-                       const void* aux, claw2d_ctrl* ctrl, int use_ctrl, double dt, hipStream_t s) {
-  int slot = 0;
-  const void* kern = step_kernel<T, P, SPLIT>(d, &slot);
-  static thread_local long cached[5] = {-1, -1, -1, -1, -1};
+                       const void* aux, claw2d_ctrl* ctrl, int use_ctrl, double dt, hipStream_t s,
+                       bool pad = false) {
+  int slot = 5;
+  const void* kern = (const void*)step2_fused<T, P, 0, false, -1, true>;
+  if (!pad || SPLIT) kern = step_kernel<T, P, SPLIT>(d, &slot);
+  static thread_local long cached[6] = {-1, -1, -1, -1, -1, -1};
   if (cached[slot] < 0) cached[slot] = resident_waves(kern);
   StepArgs<T, P> a;
   fill_args(a, d, cached[slot]);
@@ -874,22 +884,22 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
 
 template <class T, bool SPLIT>
 static int dispatch_typed(const claw2d_desc* d, void* q0, void* q1, const void* aux, claw2d_ctrl* ctrl,
-                          int use_ctrl, double dt, hipStream_t s) {
+                          int use_ctrl, double dt, hipStream_t s, bool pad) {
   switch (d->solver) {
     case CLAW_SOLVER_SHALLOW_ROE_EFIX:
-      return launch_step<T, SweP<T>, SPLIT>(d, swe_par<T>(d), q0, q1, aux, ctrl, use_ctrl, dt, s);
+      return launch_step<T, SweP<T>, SPLIT>(d, swe_par<T>(d), q0, q1, aux, ctrl, use_ctrl, dt, s, pad);
     case CLAW_SOLVER_SHALLOW_BATHY_FWAVE:
-      return launch_step<T, SweBathyP<T>, SPLIT>(d, bathy_par<T>(d), q0, q1, aux, ctrl, use_ctrl, dt, s);
+      return launch_step<T, SweBathyP<T>, SPLIT>(d, bathy_par<T>(d), q0, q1, aux, ctrl, use_ctrl, dt, s, pad);
     default:
-      return launch_step<T, EulerP<T>, SPLIT>(d, euler_par<T>(d), q0, q1, aux, ctrl, use_ctrl, dt, s);
+      return launch_step<T, EulerP<T>, SPLIT>(d, euler_par<T>(d), q0, q1, aux, ctrl, use_ctrl, dt, s, pad);
   }
 }
 
 template <bool SPLIT>
 static int dispatch_step(const claw2d_desc* d, void* q0, void* q1, const void* aux, claw2d_ctrl* ctrl,
-                         int use_ctrl, double dt, hipStream_t s) {
-  return d->dtype == CLAW_FP64 ? dispatch_typed<double, SPLIT>(d, q0, q1, aux, ctrl, use_ctrl, dt, s)
-                               : dispatch_typed<float, SPLIT>(d, q0, q1, aux, ctrl, use_ctrl, dt, s);
+                         int use_ctrl, double dt, hipStream_t s, bool pad = false) {
+  return d->dtype == CLAW_FP64 ? dispatch_typed<double, SPLIT>(d, q0, q1, aux, ctrl, use_ctrl, dt, s, pad)
+                               : dispatch_typed<float, SPLIT>(d, q0, q1, aux, ctrl, use_ctrl, dt, s, pad);
 }
 
 // The batched kernel for a method, chosen as step_kernel chooses the single-grid one, so that a
@@ -1066,6 +1076,27 @@ int claw2d_step_ds(const claw2d_desc* d, const void* q_in, void* q_out, double d
 int claw2d_advance_ds(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctrl, long long attempts,
                       void* stream) {
   return advance_entry<true>(d, q0, q1, nullptr, ctrl, attempts, stream, false, "claw2d_advance_ds");
+}
+
+int claw2d_step_padded(const claw2d_desc* d, const void* qbc_in, void* q_out, const void* aux,
+                       double dt, claw2d_ctrl* ctrl, void* stream) {
+  if (!d) return fail(-1, "null descriptor");
+  // the state's boundary conditions are in qbc_in's ghost cells; any code, custom included, is
+  // accepted for them (the aux BCs are still applied by index mapping)
+  claw2d_desc dd = *d;
+  for (int k = 0; k < 2; ++k) {
+    const int lo = dd.bc_lower[k], hi = dd.bc_upper[k];
+    if (lo < 0 || lo > 3 || hi < 0 || hi > 3) return fail(-10, "bc must be custom(0), extrap(1), periodic(2) or wall(3)");
+    dd.bc_lower[k] = dd.bc_upper[k] = CLAW_BC_EXTRAP;
+  }
+  int rc = check_desc(&dd);
+  if (rc) return rc;
+  if (!qbc_in || !q_out || !ctrl) return fail(-22, "null buffer");
+  if (qbc_in == q_out) return fail(-23, "qbc_in and q_out must differ");
+  if (!(dt > 0)) return fail(-21, "dt must be positive");
+  if (dd.num_aux > 0 && !aux) return fail(-22, "null aux buffer");
+  return dispatch_step<false>(&dd, const_cast<void*>(qbc_in), q_out, aux, ctrl, 0, dt, (hipStream_t)stream,
+                              true);
 }
 
 int claw2d_step_aux(const claw2d_desc* d, const void* q_in, void* q_out, const void* aux, double dt,

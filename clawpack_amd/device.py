@@ -171,6 +171,20 @@ class DeviceGrid:
                               C.c_void_p(dst.data_ptr()), float(dt), self.ctrl_ptr,
                               _stream_handle(stream)), "claw2d_step")
 
+    def step_padded(self, qbc, dst, dt, stream=None):
+        """One unsplit fixed-dt step from the ghost-padded planes `qbc` (num_eqn, my+4, mx+4), whose
+        ghost cells the caller has filled, into the interior planes `dst`; cur is not changed."""
+        d = self.desc
+        shape = (d.num_eqn, d.my + 4, d.mx + 4)
+        if tuple(qbc.shape) != shape or qbc.dtype != self.torch_dtype or not qbc.is_contiguous():
+            raise ValueError("step_padded needs contiguous padded planes of shape %s" % (shape,))
+        if tuple(dst.shape) != tuple(self.q[0].shape) or dst.dtype != self.torch_dtype or not dst.is_contiguous():
+            raise ValueError("step_padded: dst must be interior planes of shape %s" % (tuple(self.q[0].shape),))
+        aux = C.c_void_p(self.aux.data_ptr()) if self.aux is not None else None
+        check(self.lib.claw2d_step_padded(C.byref(d), C.c_void_p(qbc.data_ptr()), C.c_void_p(dst.data_ptr()),
+                                          aux, float(dt), self.ctrl_ptr, _stream_handle(stream)),
+              "claw2d_step_padded")
+
     def enqueue_step(self, dt, stream=None):
         """Like step() but without reading the control block back (no host synchronisation)."""
         src, dst = self.q[self.cur], self.q[1 - self.cur]

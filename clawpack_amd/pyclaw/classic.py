@@ -5,7 +5,10 @@ the GPU through libclawpack_amd.so: one fused kernel per step attempt plus a dev
 accept/reject/dt controller, so the host only synchronises at output times.  With a step_source
 or before_step callback (upstream Solver.step: Godunov or Strang source splitting) the loop runs
 on the host, one step launch per attempt, and the callbacks act on the device state through torch
-(DeviceState).  There is no CPU compute path: a missing library or device raises.
+(DeviceState).  A BC.custom side (user_bc_lower/user_bc_upper) also runs the host loop: each attempt
+fills a ghost-padded copy of the state on the device in pyclaw's order, calling the user's function
+on torch views of it, and steps from it (claw2d_step_padded).  There is no CPU compute path: a
+missing library or device raises.
 """
 import math
 
@@ -19,6 +22,25 @@ from .limiters import tvd
 from .solver import BC, Solver
 
 _KERNEL_LANGUAGES = ("HIP",)
+
+
+def _fill_side(v, axis, side, bc, g, reflect):
+    """pyclaw's _bc_lower/_bc_upper for one standard rule on the padded pyclaw-layout view v
+    (num_eqn, mx+2g, my+2g): full slices along the other axis; reflect < 0 for aux (no negation)."""
+    n = v.shape[axis]
+    for i in range(g):
+        dst = i if side == 0 else n - 1 - i
+        if bc == BC.extrap:
+            src = g if side == 0 else n - 1 - g
+        elif bc == BC.periodic:
+            src = n - 2 * g + i if side == 0 else 2 * g - 1 - i
+        elif bc == BC.wall:
+            src = 2 * g - 1 - i if side == 0 else n - 2 * g + i
+        else:
+            raise NotImplementedError("bc %r" % (bc,))
+        v.select(axis, dst).copy_(v.select(axis, src))
+        if bc == BC.wall and reflect >= 0:
+            v[reflect].select(axis - 1, dst).neg_()
 
 
 class DeviceState:
@@ -128,10 +150,18 @@ class ClawSolver2D(Solver):
             if lo is None or hi is None:
                 raise Exception("Lower and upper boundary conditions must be specified for every "
                                 "dimension (solver.bc_lower / solver.bc_upper)")
-            if BC.custom in (lo, hi):
-                raise NotImplementedError("BC.custom (user_bc) is not supported on the HIP path")
             if (lo == BC.periodic) != (hi == BC.periodic):
                 raise Exception("Periodic boundary conditions must be set on both sides")
+        custom = [BC.custom in (self.bc_lower[k], self.bc_upper[k]) for k in range(2)]
+        if any(custom):
+            if self.dimensional_split:
+                raise NotImplementedError("BC.custom with dimensional_split=True is not supported on the "
+                                          "HIP path (the split step would call the user function between "
+                                          "its sweeps); use dimensional_split=False")
+            if BC.custom in self.bc_lower and self.user_bc_lower is None:
+                raise Exception("Lower custom BC function has not been set.")
+            if BC.custom in self.bc_upper and self.user_bc_upper is None:
+                raise Exception("Upper custom BC function has not been set.")
         if self.precision not in ("fp64", "fp32"):
             raise ValueError("solver.precision must be 'fp64' or 'fp32'")
         if state.num_eqn != self.rp.num_eqn:
@@ -148,7 +178,11 @@ class ClawSolver2D(Solver):
             self.rp.kernel_id, _lib.FP64 if self.precision == "fp64" else _lib.FP32, mx, my, dx, dy,
             self.rp.num_eqn, self.rp.num_waves, order=self.order,
             transverse_waves=self.transverse_waves, mthlim=self._mthlim(),
-            bc_lower=self.bc_lower, bc_upper=self.bc_upper, reflect_index=self.reflect_index,
+            # a custom side's ghost cells come from the user's function through the padded step,
+            # which does not read the state's BC codes; the descriptor carries extrap there
+            bc_lower=[BC.extrap if b == BC.custom else b for b in self.bc_lower],
+            bc_upper=[BC.extrap if b == BC.custom else b for b in self.bc_upper],
+            reflect_index=self.reflect_index,
             grav=pd.get("grav", 1.0), gamma=pd.get("gamma", 1.4), cfl_max=self.cfl_max,
             cfl_desired=self.cfl_desired, dt_max=self.dt_max, dt_variable=self.dt_variable,
             seg_h=self.seg_h, num_aux=state.num_aux, fwave=bool(self.fwave),
@@ -182,6 +216,39 @@ class ClawSolver2D(Solver):
         return self._grid
 
     # ----------------------------------------------------------------------------- steps ---
+    def _has_custom_bc(self):
+        return BC.custom in list(self.bc_lower) + list(self.bc_upper)
+
+    def _padded(self, g):
+        """Device buffers of the padded path: qbc (num_eqn, my+4, mx+4) and, with aux, auxbc
+        (num_aux, my+4, mx+4) filled once from the aux planes with the aux BCs."""
+        d = g.desc
+        qbc = torch.empty((d.num_eqn, d.my + 4, d.mx + 4), dtype=g.torch_dtype, device=g.device)
+        auxbc = None
+        if g.aux is not None:
+            auxbc = torch.empty((d.num_aux, d.my + 4, d.mx + 4), dtype=g.torch_dtype, device=g.device)
+            auxbc[:, 2:-2, 2:-2].copy_(g.aux)
+            v = auxbc.permute(0, 2, 1)
+            for k in range(2):
+                _fill_side(v, k + 1, 0, self.aux_bc_lower[k], 2, -1)
+                _fill_side(v, k + 1, 1, self.aux_bc_upper[k], 2, -1)
+        return qbc, auxbc
+
+    def _fill_qbc(self, qbc, auxbc, src, ds, t):
+        """pyclaw's _apply_bcs on the device: interior from the planes src, then per dimension (x,
+        then y) the lower and the upper side, standard rules by slicing and BC.custom sides by the
+        user's function(state, dim, t, qbc, auxbc, num_ghost) on pyclaw-layout torch views."""
+        qbc[:, 2:-2, 2:-2].copy_(src)
+        v = qbc.permute(0, 2, 1)
+        av = None if auxbc is None else auxbc.permute(0, 2, 1)
+        ds.bind(src, t)
+        for k, dim in enumerate(ds.grid.dimensions):
+            for side, bc, fn in ((0, self.bc_lower[k], self.user_bc_lower), (1, self.bc_upper[k], self.user_bc_upper)):
+                if bc == BC.custom:
+                    fn(ds, dim, t, v, av, self.num_ghost)
+                else:
+                    _fill_side(v, k + 1, side, bc, 2, self.reflect_index[k])
+
     def step_hyperbolic(self, solution):
         """One step (step2, or step2ds when dimensional_split) with the current dt, no
         accept/reject, as ClawSolver2D.step_hyperbolic."""
@@ -192,7 +259,15 @@ class ClawSolver2D(Solver):
         g = self._device(state)
         g.upload(state.q)
         g.init_ctrl(solution.t, self.dt, target_steps=1)
-        cfl, c = g.step(self.dt)
+        if self._has_custom_bc():
+            qbc, auxbc = self._padded(g)
+            self._fill_qbc(qbc, auxbc, g.q[g.cur], DeviceState(state, g), solution.t)
+            g.step_padded(qbc, g.q[1 - g.cur], self.dt)
+            g.cur = 1 - g.cur
+            c = g.read_ctrl()
+            cfl = c.cfl
+        else:
+            cfl, c = g.step(self.dt)
         state.q[...] = g.download()
         self.cfl.update_global_max(cfl)
         if c.status & _lib.STATUS_BAD_STATE:
@@ -207,7 +282,7 @@ class ClawSolver2D(Solver):
         state = solution.state
         self._check_supported(state)
         self._check_cfl_settings()
-        if self.step_source is not None or self.before_step is not None:
+        if self.step_source is not None or self.before_step is not None or self._has_custom_bc():
             return self._evolve_with_callbacks(solution, tend)
         g = self._device(state)
         tstart = solution.t
@@ -252,10 +327,10 @@ class ClawSolver2D(Solver):
         return self.status
 
     def _evolve_with_callbacks(self, solution, tend):
-        """evolve_to_time with before_step / step_source (upstream Solver.step and
+        """evolve_to_time with before_step / step_source / BC.custom (upstream Solver.step and
         evolve_to_time): per attempt before_step, for Strang splitting a half source step, the
-        hyperbolic step, and after an accepted step the source over dt (Godunov) or dt/2
-        (Strang).  The accept/reject/dt rules are those of the device controller, applied on the
+        ghost fill (with the user's BC functions) when a side is custom, the hyperbolic step, and
+        after an accepted step the source over dt (Godunov) or dt/2 (Strang).  The accept/reject/dt rules are those of the device controller, applied on the
         host after each launch.  A rejected attempt leaves q^n untouched (the Strang half step
         works on a copy)."""
         state = solution.state
@@ -265,6 +340,8 @@ class ClawSolver2D(Solver):
         take_one_step = tend is None
         strang = self.step_source is not None and self.source_split == 2
         tmp = torch.empty_like(g.q[0]) if strang else None
+        custom = self._has_custom_bc()
+        qbc, auxbc = self._padded(g) if custom else (None, None)
         t = tstart = solution.t
         dt = self.dt
         if not take_one_step:
@@ -286,7 +363,11 @@ class ClawSolver2D(Solver):
                 tmp.copy_(src)
                 self.step_source(self, ds.bind(tmp, t), dt / 2.0)
                 src = tmp
-            g.step_into(src, g.q[1 - g.cur], dt)
+            if custom:
+                self._fill_qbc(qbc, auxbc, src, ds, t)
+                g.step_padded(qbc, g.q[1 - g.cur], dt)
+            else:
+                g.step_into(src, g.q[1 - g.cur], dt)
             c = g.read_ctrl()
             cfl = c.cfl
             bad = bool(c.status & _lib.STATUS_BAD_STATE) or cfl != cfl

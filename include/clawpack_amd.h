@@ -22,7 +22,8 @@
  * State layout on the device: structure of arrays, num_eqn planes of my rows of mx cells, x
  * fastest (plane m, row j, column i at index m*mx*my + j*mx + i), element type double (FP64) or
  * float (FP32).  Interior cells only; boundary conditions are applied by index mapping inside the
- * kernels.  The transpose from pyclaw's q(num_eqn, mx, my) happens once at upload.
+ * kernels (claw2d_step_padded instead reads a caller-filled ghost layer).  The transpose from
+ * pyclaw's q(num_eqn, mx, my) happens once at upload.
  */
 #ifndef CLAWPACK_AMD_H
 #define CLAWPACK_AMD_H
@@ -39,7 +40,7 @@ extern "C" {
 enum { CLAW_SOLVER_SHALLOW_ROE_EFIX = 0, CLAW_SOLVER_EULER_4WAVE = 1,
        CLAW_SOLVER_SHALLOW_BATHY_FWAVE = 2 };
 enum { CLAW_FP64 = 0, CLAW_FP32 = 1 };
-/* pyclaw BC codes: BC.custom=0, extrap=1, periodic=2, wall=3 (custom is rejected) */
+/* pyclaw BC codes: BC.custom=0, extrap=1, periodic=2, wall=3 (custom: claw2d_step_padded only) */
 enum { CLAW_BC_CUSTOM = 0, CLAW_BC_EXTRAP = 1, CLAW_BC_PERIODIC = 2, CLAW_BC_WALL = 3 };
 /* claw2d_ctrl.status bits */
 enum { CLAW_STATUS_BAD_STATE = 1, CLAW_STATUS_CFL_FIXED_DT = 2, CLAW_STATUS_MAX_STEPS = 4 };
@@ -148,6 +149,16 @@ int claw2d_step_ds_aux(const claw2d_desc* d, const void* q_in_dev, void* q_out_d
                        const void* aux_dev, double dt, claw2d_ctrl* ctrl_dev, void* stream);
 int claw2d_advance_ds_aux(const claw2d_desc* d, void* q0_dev, void* q1_dev, const void* aux_dev,
                           claw2d_ctrl* ctrl_dev, long long attempts, void* stream);
+
+/* One unsplit step2 from a ghost-padded state, exactly the f2py call's form: qbc_in_dev holds
+ * num_eqn planes of my+4 rows of mx+4 cells (x fastest) whose two ghost layers the caller has
+ * filled, as pyclaw's _apply_bcs fills qbc (any rule, BC.custom user callbacks included); q_out_dev
+ * receives the interior planes (num_eqn, my, mx).  The descriptor's q BC codes are not used (0 =
+ * custom is accepted); aux_dev (num_aux > 0) is as for claw2d_step_aux, its BCs by index mapping.
+ * CFL number and status as claw2d_step.  Replaces classic2.step2(..., qold, qnew, ...) called after
+ * _apply_bcs (upstream classic/solver.py step_hyperbolic, classic/step2.f90). */
+int claw2d_step_padded(const claw2d_desc* d, const void* qbc_in_dev, void* q_out_dev,
+                       const void* aux_dev, double dt, claw2d_ctrl* ctrl_dev, void* stream);
 
 /* Bytes of the workspace claw2d_mass_sum needs. */
 size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc* d);

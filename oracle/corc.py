@@ -285,14 +285,53 @@ def component_sum(d, q, m):
     return lib().orc_component_sum(C.byref(d), _p(q), m)
 
 
+def apply_bcs(d, c, q, t=0.0, user_bc_lower=None, user_bc_upper=None, auxbc=None):
+    """Solver._apply_bcs (upstream pyclaw/src/pyclaw/solver.py, absent) in numpy: q copied into
+    qbc (num_eqn, mx+2g, my+2g), then for x and then y the lower and the upper side, each over full
+    slices of the other dimension: extrap copies the edge cell, periodic wraps, wall mirrors and
+    negates the reflect_index component, custom (code 0) calls the user function
+    f(None, idim, t, qbc, auxbc, num_ghost), which fills that side of qbc in place."""
+    g = d.num_ghost
+    qbc = np.zeros((d.num_eqn, d.mx + 2 * g, d.my + 2 * g), order="F")
+    qbc[:, g:-g, g:-g] = q
+    for idim in range(2):
+        a = np.moveaxis(qbc, idim + 1, 1)  # a view, with this dimension as axis 1
+        ri = c.reflect_index[idim]
+        for side, bc in ((0, c.bc_lower[idim]), (1, c.bc_upper[idim])):
+            if bc == BC_CUSTOM:
+                (user_bc_lower if side == 0 else user_bc_upper)(None, idim, t, qbc, auxbc, g)
+            elif side == 0:
+                if bc == BC_EXTRAP:
+                    for i in range(g):
+                        a[:, i] = a[:, g]
+                elif bc == BC_PERIODIC:
+                    a[:, :g] = a[:, -2 * g:-g]
+                elif bc == BC_WALL:
+                    for i in range(g):
+                        a[:, i] = a[:, 2 * g - 1 - i]
+                        a[ri, i] = -a[ri, 2 * g - 1 - i]
+            else:
+                if bc == BC_EXTRAP:
+                    for i in range(g):
+                        a[:, -i - 1] = a[:, -g - 1]
+                elif bc == BC_PERIODIC:
+                    a[:, -g:] = a[:, g:2 * g]
+                elif bc == BC_WALL:
+                    for i in range(g):
+                        a[:, -i - 1] = a[:, -2 * g + i]
+                        a[ri, -i - 1] = -a[ri, -2 * g + i]
+    return qbc
+
+
 def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, nsteps=None,
-                       tend=None, t0=0.0, dt=None, aux=None):
+                       tend=None, t0=0.0, dt=None, aux=None, user_bc_lower=None, user_bc_upper=None):
     """Solver.evolve_to_time with Solver.step's source splitting (upstream pyclaw/src/pyclaw/
     solver.py, absent from the reference tree): per attempt before_step(q, t), for Strang
     splitting (source_split 2) a half step source(q, dt/2) before the hyperbolic step, and after
     an accepted step source(q, dt) (Godunov, source_split 1) or source(q, dt/2) (Strang); a
     rejected attempt restores q and t.  source and before_step act on q (num_eqn, mx, my) in
-    place.  The accept/reject, t and dt rules are orc_evolve's; nsteps stops after that many
+    place.  With custom BC codes (0) in c the ghost cells come from apply_bcs with the user
+    functions and the step is orc_step2 on that qbc (unsplit).  The accept/reject, t and dt rules are orc_evolve's; nsteps stops after that many
     accepted steps (no clipping to tend), tend runs to that time.  The hyperbolic step is
     orc_step_hyperbolic."""
     q = np.array(q, dtype=np.float64, order="F", copy=True)
@@ -311,7 +350,14 @@ def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, n
             before_step(q, t)
         if source is not None and source_split == 2:
             source(q, dt / 2.0)
-        qn, cfl = step_hyperbolic(d, c, q, dt, aux=aux)
+        if BC_CUSTOM in tuple(c.bc_lower) + tuple(c.bc_upper):
+            g = d.num_ghost
+            auxbc = None if aux is None else fill_aux_ghosts(d, c, aux)
+            qbc = apply_bcs(d, c, q, t, user_bc_lower, user_bc_upper, auxbc=auxbc)
+            qnbc, cfl = step2(d, qbc, dt, auxbc=auxbc)
+            qn = np.asfortranarray(qnbc[:, g:-g, g:-g])
+        else:
+            qn, cfl = step_hyperbolic(d, c, q, dt, aux=aux)
         st["attempts"] += 1
         if not (cfl > c.cfl_max):
             q = qn

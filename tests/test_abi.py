@@ -141,3 +141,20 @@ def test_entry_points_without_aux_refuse_aux_solvers():
     assert lib.claw2d_step_ds_aux(C.byref(d), qa, qb, aux, -0.1, ctrl, None) == -21
     # batched grids and the adjoint take no aux solvers
     assert lib.claw2d_adjoint_workspace_bytes(C.byref(d)) == 0
+
+
+def test_step_padded_accepts_custom_codes_and_checks_arguments():
+    """claw2d_step_padded takes the state's ghost cells from its input, so BC code 0 (custom) is
+    accepted there and only there; argument errors are refused before any device call."""
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    d = _desc(bc_lower=(0, 1), bc_upper=(1, 0))
+    assert lib.claw2d_validate(C.byref(d)) == -10
+    qa, qb, ctrl = C.c_void_p(4096), C.c_void_p(8192), C.c_void_p(12288)
+    assert lib.claw2d_step_padded(C.byref(d), None, qb, None, 0.1, ctrl, None) == -22
+    assert lib.claw2d_step_padded(C.byref(d), qa, qa, None, 0.1, ctrl, None) == -23
+    assert lib.claw2d_step_padded(C.byref(d), qa, qb, None, 0.0, ctrl, None) == -21
+    bad = _desc(bc_lower=(7, 1))
+    assert lib.claw2d_step_padded(C.byref(bad), qa, qb, None, 0.1, ctrl, None) == -10
+    aux_d = _desc(solver_id=2, num_aux=1, fwave=1, bc_lower=(0, 3))
+    assert lib.claw2d_step_padded(C.byref(aux_d), qa, qb, None, 0.1, ctrl, None) == -22

@@ -62,8 +62,18 @@ def test_unsupported_configurations_raise(attr, value, exc):
 def test_bc_checks():
     solver, state, domain, sol = make()
     solver.bc_lower = [pyclaw.BC.custom, pyclaw.BC.extrap]
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(Exception, match="Lower custom BC function has not been set"):
         solver.setup(sol)
+    solver.user_bc_lower = lambda state, dim, t, qbc, auxbc, num_ghost: None
+    solver.dimensional_split = True
+    with pytest.raises(NotImplementedError, match="dimensional_split"):
+        solver.setup(sol)
+    solver.dimensional_split = False
+    solver.bc_lower = [pyclaw.BC.extrap, pyclaw.BC.extrap]
+    solver.bc_upper = [pyclaw.BC.extrap, pyclaw.BC.custom]
+    with pytest.raises(Exception, match="Upper custom BC function has not been set"):
+        solver.setup(sol)
+    solver.bc_upper = [pyclaw.BC.extrap, pyclaw.BC.extrap]
     solver.bc_lower = [pyclaw.BC.periodic, pyclaw.BC.extrap]
     with pytest.raises(Exception, match="Periodic"):
         solver.setup(sol)

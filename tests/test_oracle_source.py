@@ -76,3 +76,63 @@ def test_strang_half_steps_and_rejected_attempts():
     m0 = corc.component_sum(d, q, 1)
     m1 = corc.component_sum(d, qn, 1)
     assert abs(m1 - m0 * np.exp(-k * st["t"])) <= 1e-12 * abs(m0)
+
+
+# ------------------------------------------------------------------------------------------------
+# BC.custom: the numpy restatement of _apply_bcs with user functions
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("bcl,bcu,ri", [((1, 1), (1, 1), (1, 2)), ((2, 3), (2, 3), (1, 2)),
+                                         ((3, 1), (3, 2), (2, 1)), ((3, 3), (1, 3), (1, 2))])
+def test_apply_bcs_standard_rules_equal_orc_fill_ghosts(bcl, bcu, ri):
+    if (bcl[1] == 2) != (bcu[1] == 2):
+        bcu = (bcu[0], bcl[1])
+    rng = np.random.default_rng(1)
+    mx, my = 9, 7
+    q = np.asfortranarray(rng.standard_normal((3, mx, my)))
+    d = corc.make_desc("swe", mx, my, 0.1, 0.1)
+    c = corc.make_ctrl(bc_lower=bcl, bc_upper=bcu, reflect_index=ri)
+    assert np.array_equal(corc.apply_bcs(d, c, q), corc.fill_ghosts(d, c, q))
+
+
+def extrap_user(side):
+    """A user BC function that writes the extrapolation rule itself (x or y, lower or upper)."""
+    def fn(state, idim, t, qbc, auxbc, g):
+        a = np.moveaxis(qbc, idim + 1, 1)
+        for i in range(g):
+            if side == 0:
+                a[:, i] = a[:, g]
+            else:
+                a[:, -i - 1] = a[:, -g - 1]
+    return fn
+
+
+def test_custom_function_reproducing_extrap_gives_the_extrap_run():
+    q, dx, dy = problems.radial_dam_break(40, 30)
+    d = corc.make_desc("swe", 40, 30, dx, dy, grav=1.0)
+    ref, sr = corc.evolve(d, corc.make_ctrl(bc_lower=(1, 3), bc_upper=(1, 3)), q, tend=0.4)
+    c = corc.make_ctrl(bc_lower=(0, 3), bc_upper=(0, 3))
+    got, sg = corc.evolve_with_source(d, c, q, tend=0.4, user_bc_lower=extrap_user(0),
+                                      user_bc_upper=extrap_user(1))
+    assert sg["numsteps"] == sr["numsteps"] and sg["t"] == sr["t"]
+    assert np.array_equal(got, ref)
+
+
+def test_custom_inflow_boundary_sends_a_bore():
+    """An inflow state h = 1.5, hu = 0.6 imposed in the left ghost columns of still water h = 1
+    drives a bore into the domain: mass grows by about the inflow rate times t, and the far right
+    stays at rest."""
+    mx, my = 80, 12
+    q = np.zeros((3, mx, my), order="F")
+    q[0] = 1.0
+    d = corc.make_desc("swe", mx, my, 1.0 / mx, 1.0 / my, grav=1.0)
+    c = corc.make_ctrl(bc_lower=(0, 3), bc_upper=(1, 3))
+
+    def inflow(state, idim, t, qbc, auxbc, g):
+        qbc[0, :g, :] = 1.5
+        qbc[1, :g, :] = 0.6
+        qbc[2, :g, :] = 0.0
+
+    qn, st = corc.evolve_with_source(d, c, q, tend=0.3, user_bc_lower=inflow)
+    mass0, mass1 = q[0].sum() / (mx * my), qn[0].sum() / (mx * my)
+    assert 0.5 * 0.6 * 0.3 < mass1 - mass0 < 1.5 * 0.6 * 0.3
+    assert np.array_equal(qn[:, -10:], q[:, -10:])
