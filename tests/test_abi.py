@@ -156,5 +156,8 @@ def test_step_padded_accepts_custom_codes_and_checks_arguments():
     assert lib.claw2d_step_padded(C.byref(d), qa, qb, None, 0.0, ctrl, None) == -21
     bad = _desc(bc_lower=(7, 1))
     assert lib.claw2d_step_padded(C.byref(bad), qa, qb, None, 0.1, ctrl, None) == -10
-    aux_d = _desc(solver_id=2, num_aux=1, fwave=1, bc_lower=(0, 3))
-    assert lib.claw2d_step_padded(C.byref(aux_d), qa, qb, None, 0.1, ctrl, None) == -22
+    aux_d = _desc(solver_id=2, num_aux=1, fwave=1, bc_lower=(0, 3), aux_bc_lower=(1, 3), aux_bc_upper=(1, 1))
+    assert lib.claw2d_step_padded(C.byref(aux_d), qa, qb, None, 0.1, ctrl, None) == -22  # null aux
+    # aux boundary conditions are still index maps: custom is refused for them
+    aux_custom = _desc(solver_id=2, num_aux=1, fwave=1, bc_lower=(0, 3), aux_bc_lower=(0, 3), aux_bc_upper=(1, 1))
+    assert lib.claw2d_step_padded(C.byref(aux_custom), qa, qb, C.c_void_p(16384), 0.1, ctrl, None) == -19
