@@ -66,6 +66,7 @@ SIGNATURES = {
     "claw2d_step_ds_aux": (C.c_int, [_PD, _VP, _VP, _VP, C.c_double, _VP, _VP]),
     "claw2d_advance_ds_aux": (C.c_int, [_PD, _VP, _VP, _VP, _VP, C.c_longlong, _VP]),
     "claw2d_step_padded": (C.c_int, [_PD, _VP, _VP, _VP, C.c_double, _VP, _VP]),
+    "claw2d_sweep_padded": (C.c_int, [_PD, _VP, _VP, _VP, C.c_double, C.c_int32, _VP, _VP]),
     "claw2d_mass_sum_workspace_bytes": (C.c_size_t, [_PD]),
     "claw2d_mass_sum": (C.c_int, [_PD, _VP, C.c_int32, _VP, _VP, _VP]),
     "claw2d_adjoint_workspace_bytes": (C.c_size_t, [_PD]),

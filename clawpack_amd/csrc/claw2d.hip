@@ -222,11 +222,15 @@ template <int NEQ, class T> __device__ __forceinline__ void yswap(T v[NEQ]) {
 // LIM > 0: every wave uses limiter LIM (compile-time); LIM == 0: per-wave ids from a.mthlim.
 // PAD: qin is a ghost-padded state (num_eqn planes of my+4 rows of mx+4 cells, ghost cells filled
 // by the caller, as classic step2 receives qbc); q's boundary conditions are then not applied here.
-template <class T, class P, int LIM, bool SPLIT = false, int TWC = -1, bool PAD = false>
+// SWEEP (split, padded): 0 both sweeps in one pass; 1 the x-sweep only, q* written into the
+// interior of the padded qout (its ghost cells are left to the caller); 2 the y-sweep only, from a
+// padded q* into the interior planes qout.
+template <class T, class P, int LIM, bool SPLIT = false, int TWC = -1, bool PAD = false, int SWEEP = 0>
 __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __restrict__ qin,
                                            T* __restrict__ qout, const int mx, const int my,
                                            const int x0, const int y0, const int y1, const double dt,
                                            const double dx, const double dy, claw2d_ctrl* ctrl) {
+  static_assert(SWEEP == 0 || (SPLIT && PAD), "single sweeps run on the padded split path");
   constexpr int NEQ = P::NEQ, NW = P::NW, NC = P::NC, NL = P::NL;
   // 4 equations: add each y-interface's upper-cell shares at once (two more exchanges per row)
   // instead of carrying them to the next row, so the kernel fits 2 wavefronts per SIMD
@@ -302,22 +306,24 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
 #pragma unroll
   for (int p = 0; p < NW; ++p) yal2[p] = ymprev[p] = T(0);
 
+  // the x-sweep alone needs only the owned rows; every other form marches two halo rows each side
+  const int rlo = SWEEP == 1 ? y0 : y0 - 2, rhi = SWEEP == 1 ? y1 - 1 : y1 + 1;
   bool fnext;
   T bnext = T(0);
-  load_row(y0 - 2, qnext, fnext, bnext);
-  for (int R = y0 - 2; R <= y1 + 1; ++R) {
+  load_row(rlo, qnext, fnext, bnext);
+  for (int R = rlo; R <= rhi; ++R) {
     T qR[NEQ];
 #pragma unroll
     for (int m = 0; m < NEQ; ++m) qR[m] = qnext[m];
     const T bR = bnext;
     apply_flips(qR, fnext);
-    if (R <= y1) load_row(R + 1, qnext, fnext, bnext);
+    if (R < rhi) load_row(R + 1, qnext, fnext, bnext);
     const Cell cR = prep_cell<P>(qR, bR, a.par);
 
     // ---- request the left neighbour's cell data for the x-interfaces of row R now, so that the
     // y work below hides the ds_bpermute latency
     T pl[NC];
-    {
+    if constexpr (SWEEP != 2) {
       T pk[NC];
       P::pack(cR, pk);
 #pragma unroll
@@ -407,7 +413,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
     // their mapped source rows, which equals refilling the ghosts of q* for extrap, periodic and
     // wall conditions.
     Cell cY = cR;
-    if (SPLIT || (R >= y0 - 1 && R <= y1)) {
+    if (SWEEP != 2 && (SPLIT || (R >= y0 - 1 && R <= y1))) {
       const Cell cL = P::unpack(pl);
       Iface fxi;
       P::template rpn2<0>(cL, cR, a.par, fxi);
@@ -437,7 +443,15 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         T qs[NEQ];
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) qs[m] = qR[m] + (RX[m] + shfl(LX[m], addrR));
-        cY = prep_cell<P>(qs, bR, a.par);
+        if constexpr (SWEEP == 1) {
+          if (own_col) {
+            const size_t off = (size_t)(R + 2) * (size_t)(mx + 4) + (size_t)(c + 2);
+#pragma unroll
+            for (int m = 0; m < NEQ; ++m) qout[(size_t)m * plane_in + off] = qs[m];
+          }
+        } else {
+          cY = prep_cell<P>(qs, bR, a.par);
+        }
       } else {
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) {
@@ -450,19 +464,21 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         }
       }
     }
-    if constexpr (SPLIT) y_part(cY);
+    if constexpr (SPLIT && SWEEP != 1) y_part(cY);
 
-    // ---- roll the column window
+    // ---- roll the column window (the x-sweep alone keeps no column state)
+    if constexpr (SWEEP != 1) {
 #pragma unroll
-    for (int m = 0; m < NEQ; ++m) {
-      qm2[m] = cprev.q[m];  // row R-1 (cprev still holds it)
-      acc0[m] = acc1[m];
-      acc1[m] = acc2[m];
-      acc2[m] = acc3[m];
-      acc3[m] = T(0);
+      for (int m = 0; m < NEQ; ++m) {
+        qm2[m] = cprev.q[m];  // row R-1 (cprev still holds it)
+        acc0[m] = acc1[m];
+        acc1[m] = acc2[m];
+        acc2[m] = acc3[m];
+        acc3[m] = T(0);
+      }
+      cprev = cY;
+      fyprev = fynew;
     }
-    cprev = cY;
-    fyprev = fynew;
   }
 
   // Wave-uniform epilogue: every lane issues the atomics with the same operand (the compiler
@@ -572,7 +588,7 @@ template <class T> struct MinWaves {
   static constexpr int value = sizeof(T) == 8 ? CLAW_MIN_WAVES_F64 : CLAW_MIN_WAVES_F32;
 };
 
-template <class T, class P, int LIM, bool SPLIT, int TWC, bool PAD = false>
+template <class T, class P, int LIM, bool SPLIT, int TWC, bool PAD = false, int SWEEP = 0>
 __global__ void __launch_bounds__(kLanes* kWPB) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::value)))
 step2_fused(StepArgs<T, P> a) {
   double dt = a.dt_fixed;
@@ -589,7 +605,7 @@ step2_fused(StepArgs<T, P> a) {
   const int strip = wave % a.nstrips, seg = wave / a.nstrips;
   const int x0 = strip * kOwn, y0 = seg * a.seg_h;
   const int y1 = min(y0 + a.seg_h, a.my);
-  march_item<T, P, LIM, SPLIT, TWC, PAD>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
+  march_item<T, P, LIM, SPLIT, TWC, PAD, SWEEP>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
   if (a.fuse) {
     // The last wavefront to finish closes the step: its speed maxima are then complete.  The
     // wait for this wavefront's outstanding memory operations (its max/or atomics are acknowledged
@@ -882,6 +898,52 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   return e == hipSuccess ? 0 : hip_fail(e, "step2_fused launch");
 }
 
+// One sweep of the split step from a caller-filled padded state (the general-limiter kernel).
+// Sweep 1 folds its x-speed maximum into the control block and leaves the step open; sweep 2 adds
+// the y-speed maximum and closes it, so ctrl->cfl is the maximum over both sweeps.
+template <class T, class P>
+static int launch_sweep(const claw2d_desc* d, const typename P::Par& par, const void* qbc, void* qout,
+                        const void* aux, claw2d_ctrl* ctrl, double dt, int ids, hipStream_t s) {
+  const void* kern = ids == 1 ? (const void*)step2_fused<T, P, 0, true, -1, true, 1>
+                              : (const void*)step2_fused<T, P, 0, true, -1, true, 2>;
+  static thread_local long cached[2] = {-1, -1};
+  long& res = cached[ids - 1];
+  if (res < 0) res = resident_waves(kern);
+  StepArgs<T, P> a;
+  fill_args(a, d, res);
+  a.q0 = (T*)qbc;
+  a.q1 = (T*)qout;
+  a.ctrl = ctrl;
+  a.use_ctrl = 0;
+  a.dt_fixed = dt;
+  a.par = par;
+  a.fuse = ids == 1 ? 0 : 1;
+  a.cfl_max = d->cfl_max;
+  a.cfl_desired = d->cfl_desired;
+  a.dt_max = d->dt_max;
+  a.tw = 0;
+  a.aux = (const T*)aux;
+  const int waves = a.nstrips * a.nsegs;
+  a.total_waves = waves;
+  const int blocks = (waves + kWPB - 1) / kWPB;
+  void* args[] = {&a};
+  const hipError_t e = hipLaunchKernel(kern, dim3(blocks), dim3(kLanes * kWPB), args, 0, s);
+  return e == hipSuccess ? 0 : hip_fail(e, "step2_fused sweep launch");
+}
+
+template <class T>
+static int dispatch_sweep(const claw2d_desc* d, const void* qbc, void* qout, const void* aux,
+                          claw2d_ctrl* ctrl, double dt, int ids, hipStream_t s) {
+  switch (d->solver) {
+    case CLAW_SOLVER_SHALLOW_ROE_EFIX:
+      return launch_sweep<T, SweP<T>>(d, swe_par<T>(d), qbc, qout, aux, ctrl, dt, ids, s);
+    case CLAW_SOLVER_SHALLOW_BATHY_FWAVE:
+      return launch_sweep<T, SweBathyP<T>>(d, bathy_par<T>(d), qbc, qout, aux, ctrl, dt, ids, s);
+    default:
+      return launch_sweep<T, EulerP<T>>(d, euler_par<T>(d), qbc, qout, aux, ctrl, dt, ids, s);
+  }
+}
+
 template <class T, bool SPLIT>
 static int dispatch_typed(const claw2d_desc* d, void* q0, void* q1, const void* aux, claw2d_ctrl* ctrl,
                           int use_ctrl, double dt, hipStream_t s, bool pad) {
@@ -1078,18 +1140,44 @@ int claw2d_advance_ds(const claw2d_desc* d, void* q0, void* q1, claw2d_ctrl* ctr
   return advance_entry<true>(d, q0, q1, nullptr, ctrl, attempts, stream, false, "claw2d_advance_ds");
 }
 
+}  // extern "C"
+
+namespace claw {
+// Checks shared by the padded entry points.  The state's boundary conditions are in the caller's
+// ghost cells, so any code, custom included, is accepted for them; dd gets extrap in their place
+// (the aux BCs are still applied by index mapping).
+static int padded_desc(const claw2d_desc* d, claw2d_desc* dd) {
+  if (!d) return fail(-1, "null descriptor");
+  *dd = *d;
+  for (int k = 0; k < 2; ++k) {
+    const int lo = dd->bc_lower[k], hi = dd->bc_upper[k];
+    if (lo < 0 || lo > 3 || hi < 0 || hi > 3) return fail(-10, "bc must be custom(0), extrap(1), periodic(2) or wall(3)");
+    dd->bc_lower[k] = dd->bc_upper[k] = CLAW_BC_EXTRAP;
+  }
+  return check_desc(dd);
+}
+}  // namespace claw
+
+extern "C" {
+
+int claw2d_sweep_padded(const claw2d_desc* d, const void* qbc_in, void* q_out, const void* aux,
+                        double dt, int32_t ids, claw2d_ctrl* ctrl, void* stream) {
+  claw2d_desc dd;
+  int rc = padded_desc(d, &dd);
+  if (rc) return rc;
+  if (ids != 1 && ids != 2) return fail(-27, "ids must be 1 (x-sweeps) or 2 (y-sweeps)");
+  if (!qbc_in || !q_out || !ctrl) return fail(-22, "null buffer");
+  if (qbc_in == q_out) return fail(-23, "qbc_in and q_out must differ");
+  if (!(dt > 0)) return fail(-21, "dt must be positive");
+  if (dd.num_aux > 0 && !aux) return fail(-22, "null aux buffer");
+  return dd.dtype == CLAW_FP64 ? dispatch_sweep<double>(&dd, qbc_in, q_out, aux, ctrl, dt, ids, (hipStream_t)stream)
+                               : dispatch_sweep<float>(&dd, qbc_in, q_out, aux, ctrl, dt, ids, (hipStream_t)stream);
+}
+
 int claw2d_step_padded(const claw2d_desc* d, const void* qbc_in, void* q_out, const void* aux,
                        double dt, claw2d_ctrl* ctrl, void* stream) {
-  if (!d) return fail(-1, "null descriptor");
-  // the state's boundary conditions are in qbc_in's ghost cells; any code, custom included, is
-  // accepted for them (the aux BCs are still applied by index mapping)
-  claw2d_desc dd = *d;
-  for (int k = 0; k < 2; ++k) {
-    const int lo = dd.bc_lower[k], hi = dd.bc_upper[k];
-    if (lo < 0 || lo > 3 || hi < 0 || hi > 3) return fail(-10, "bc must be custom(0), extrap(1), periodic(2) or wall(3)");
-    dd.bc_lower[k] = dd.bc_upper[k] = CLAW_BC_EXTRAP;
-  }
-  int rc = check_desc(&dd);
+  claw2d_desc dd;
+  int rc = padded_desc(d, &dd);
   if (rc) return rc;
   if (!qbc_in || !q_out || !ctrl) return fail(-22, "null buffer");
   if (qbc_in == q_out) return fail(-23, "qbc_in and q_out must differ");

@@ -185,6 +185,23 @@ class DeviceGrid:
                                           aux, float(dt), self.ctrl_ptr, _stream_handle(stream)),
               "claw2d_step_padded")
 
+    def sweep_padded(self, qbc, dst, dt, ids, stream=None):
+        """One sweep of the split step from the ghost-padded planes `qbc` (num_eqn, my+4, mx+4):
+        ids 1 writes q* into the interior of the padded `dst` (its ghost cells are the caller's to
+        fill), ids 2 writes the interior planes `dst`.  ids 1 leaves the CFL accumulation open and
+        ids 2 closes it (ctrl.cfl = max over both sweeps); cur is not changed."""
+        d = self.desc
+        shape = (d.num_eqn, d.my + 4, d.mx + 4)
+        if tuple(qbc.shape) != shape or qbc.dtype != self.torch_dtype or not qbc.is_contiguous():
+            raise ValueError("sweep_padded needs contiguous padded planes of shape %s" % (shape,))
+        out_shape = shape if ids == 1 else tuple(self.q[0].shape)
+        if tuple(dst.shape) != out_shape or dst.dtype != self.torch_dtype or not dst.is_contiguous():
+            raise ValueError("sweep_padded(ids=%d): dst must have shape %s" % (ids, out_shape))
+        aux = C.c_void_p(self.aux.data_ptr()) if self.aux is not None else None
+        check(self.lib.claw2d_sweep_padded(C.byref(d), C.c_void_p(qbc.data_ptr()), C.c_void_p(dst.data_ptr()),
+                                           aux, float(dt), int(ids), self.ctrl_ptr, _stream_handle(stream)),
+              "claw2d_sweep_padded")
+
     def enqueue_step(self, dt, stream=None):
         """Like step() but without reading the control block back (no host synchronisation)."""
         src, dst = self.q[self.cur], self.q[1 - self.cur]

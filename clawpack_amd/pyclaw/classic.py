@@ -7,8 +7,9 @@ or before_step callback (upstream Solver.step: Godunov or Strang source splittin
 on the host, one step launch per attempt, and the callbacks act on the device state through torch
 (DeviceState).  A BC.custom side (user_bc_lower/user_bc_upper) also runs the host loop: each attempt
 fills a ghost-padded copy of the state on the device in pyclaw's order, calling the user's function
-on torch views of it, and steps from it (claw2d_step_padded).  There is no CPU compute path: a
-missing library or device raises.
+on torch views of it, and steps from it (claw2d_step_padded; with dimensional_split the two
+sweeps of claw2d_sweep_padded, with the same ghost fill of q* between them).  There is no CPU
+compute path: a missing library or device raises.
 """
 import math
 
@@ -78,7 +79,7 @@ class ClawSolver2D(Solver):
 
     Supported on the HIP path: dimensional_split True (step2ds, Godunov x-then-y splitting, the
     default) or False (unsplit step2 with transverse_waves 0/1/2), order 1 or 2, limiters 0-5 per
-    wave, BC extrap/periodic/wall, the Riemann plugins shallow_roe_with_efix_2D,
+    wave, BC extrap/periodic/wall/custom (user_bc_lower/upper on torch views), the Riemann plugins shallow_roe_with_efix_2D,
     euler_4wave_2D and shallow_bathymetry_fwave_2D (with fwave = True, state.aux[0] = bathymetry
     and solver.aux_bc_lower/upper), solver.precision 'fp64' (default) or 'fp32'.
     """
@@ -154,10 +155,6 @@ class ClawSolver2D(Solver):
                 raise Exception("Periodic boundary conditions must be set on both sides")
         custom = [BC.custom in (self.bc_lower[k], self.bc_upper[k]) for k in range(2)]
         if any(custom):
-            if self.dimensional_split:
-                raise NotImplementedError("BC.custom with dimensional_split=True is not supported on the "
-                                          "HIP path (the split step would call the user function between "
-                                          "its sweeps); use dimensional_split=False")
             if BC.custom in self.bc_lower and self.user_bc_lower is None:
                 raise Exception("Lower custom BC function has not been set.")
             if BC.custom in self.bc_upper and self.user_bc_upper is None:
@@ -220,10 +217,12 @@ class ClawSolver2D(Solver):
         return BC.custom in list(self.bc_lower) + list(self.bc_upper)
 
     def _padded(self, g):
-        """Device buffers of the padded path: qbc (num_eqn, my+4, mx+4) and, with aux, auxbc
-        (num_aux, my+4, mx+4) filled once from the aux planes with the aux BCs."""
+        """Device buffers of the padded path: qbc (num_eqn, my+4, mx+4), with dimensional_split a
+        second one for q*, and, with aux, auxbc (num_aux, my+4, mx+4) filled once from the aux
+        planes with the aux BCs."""
         d = g.desc
         qbc = torch.empty((d.num_eqn, d.my + 4, d.mx + 4), dtype=g.torch_dtype, device=g.device)
+        qsbc = torch.empty_like(qbc) if self.dimensional_split else None
         auxbc = None
         if g.aux is not None:
             auxbc = torch.empty((d.num_aux, d.my + 4, d.mx + 4), dtype=g.torch_dtype, device=g.device)
@@ -232,13 +231,17 @@ class ClawSolver2D(Solver):
             for k in range(2):
                 _fill_side(v, k + 1, 0, self.aux_bc_lower[k], 2, -1)
                 _fill_side(v, k + 1, 1, self.aux_bc_upper[k], 2, -1)
-        return qbc, auxbc
+        return qbc, qsbc, auxbc
 
     def _fill_qbc(self, qbc, auxbc, src, ds, t):
-        """pyclaw's _apply_bcs on the device: interior from the planes src, then per dimension (x,
-        then y) the lower and the upper side, standard rules by slicing and BC.custom sides by the
-        user's function(state, dim, t, qbc, auxbc, num_ghost) on pyclaw-layout torch views."""
-        qbc[:, 2:-2, 2:-2].copy_(src)
+        """pyclaw's _apply_bcs on the device: interior from the planes src (None: qbc's interior
+        already holds the state), then per dimension (x, then y) the lower and the upper side,
+        standard rules by slicing and BC.custom sides by the user's function(state, dim, t, qbc,
+        auxbc, num_ghost) on pyclaw-layout torch views."""
+        if src is None:
+            src = qbc[:, 2:-2, 2:-2]
+        else:
+            qbc[:, 2:-2, 2:-2].copy_(src)
         v = qbc.permute(0, 2, 1)
         av = None if auxbc is None else auxbc.permute(0, 2, 1)
         ds.bind(src, t)
@@ -248,6 +251,19 @@ class ClawSolver2D(Solver):
                     fn(ds, dim, t, v, av, self.num_ghost)
                 else:
                     _fill_side(v, k + 1, side, bc, 2, self.reflect_index[k])
+
+    def _custom_step(self, g, bufs, src, ds, t, dt):
+        """The hyperbolic step with ghost cells from _apply_bcs (user functions included), into the
+        other state buffer: step2 from the filled qbc, or with dimensional_split the x-sweeps into
+        q*, the same ghost fill of q* (same rules, same t) and the y-sweeps."""
+        qbc, qsbc, auxbc = bufs
+        self._fill_qbc(qbc, auxbc, src, ds, t)
+        if not self.dimensional_split:
+            g.step_padded(qbc, g.q[1 - g.cur], dt)
+            return
+        g.sweep_padded(qbc, qsbc, dt, 1)
+        self._fill_qbc(qsbc, auxbc, None, ds, t)
+        g.sweep_padded(qsbc, g.q[1 - g.cur], dt, 2)
 
     def step_hyperbolic(self, solution):
         """One step (step2, or step2ds when dimensional_split) with the current dt, no
@@ -260,9 +276,7 @@ class ClawSolver2D(Solver):
         g.upload(state.q)
         g.init_ctrl(solution.t, self.dt, target_steps=1)
         if self._has_custom_bc():
-            qbc, auxbc = self._padded(g)
-            self._fill_qbc(qbc, auxbc, g.q[g.cur], DeviceState(state, g), solution.t)
-            g.step_padded(qbc, g.q[1 - g.cur], self.dt)
+            self._custom_step(g, self._padded(g), g.q[g.cur], DeviceState(state, g), solution.t, self.dt)
             g.cur = 1 - g.cur
             c = g.read_ctrl()
             cfl = c.cfl
@@ -341,7 +355,7 @@ class ClawSolver2D(Solver):
         strang = self.step_source is not None and self.source_split == 2
         tmp = torch.empty_like(g.q[0]) if strang else None
         custom = self._has_custom_bc()
-        qbc, auxbc = self._padded(g) if custom else (None, None)
+        bufs = self._padded(g) if custom else None
         t = tstart = solution.t
         dt = self.dt
         if not take_one_step:
@@ -364,8 +378,7 @@ class ClawSolver2D(Solver):
                 self.step_source(self, ds.bind(tmp, t), dt / 2.0)
                 src = tmp
             if custom:
-                self._fill_qbc(qbc, auxbc, src, ds, t)
-                g.step_padded(qbc, g.q[1 - g.cur], dt)
+                self._custom_step(g, bufs, src, ds, t, dt)
             else:
                 g.step_into(src, g.q[1 - g.cur], dt)
             c = g.read_ctrl()

@@ -40,7 +40,7 @@ extern "C" {
 enum { CLAW_SOLVER_SHALLOW_ROE_EFIX = 0, CLAW_SOLVER_EULER_4WAVE = 1,
        CLAW_SOLVER_SHALLOW_BATHY_FWAVE = 2 };
 enum { CLAW_FP64 = 0, CLAW_FP32 = 1 };
-/* pyclaw BC codes: BC.custom=0, extrap=1, periodic=2, wall=3 (custom: claw2d_step_padded only) */
+/* pyclaw BC codes: BC.custom=0, extrap=1, periodic=2, wall=3 (custom: the padded entry points only) */
 enum { CLAW_BC_CUSTOM = 0, CLAW_BC_EXTRAP = 1, CLAW_BC_PERIODIC = 2, CLAW_BC_WALL = 3 };
 /* claw2d_ctrl.status bits */
 enum { CLAW_STATUS_BAD_STATE = 1, CLAW_STATUS_CFL_FIXED_DT = 2, CLAW_STATUS_MAX_STEPS = 4 };
@@ -159,6 +159,21 @@ int claw2d_advance_ds_aux(const claw2d_desc* d, void* q0_dev, void* q1_dev, cons
  * _apply_bcs (upstream classic/solver.py step_hyperbolic, classic/step2.f90). */
 int claw2d_step_padded(const claw2d_desc* d, const void* qbc_in_dev, void* q_out_dev,
                        const void* aux_dev, double dt, claw2d_ctrl* ctrl_dev, void* stream);
+
+/* One sweep of the dimensionally split step from a ghost-padded state, for ghost cells filled by
+ * the caller between the sweeps (BC.custom with dimensional_split).  ids = 1: x-sweeps of
+ * qbc_in_dev (padded as for claw2d_step_padded) write q* into the interior cells of q_out_dev,
+ * which is padded the same way (its ghost cells are not written: the caller fills them from q*);
+ * ids = 2: y-sweeps of the padded q* in qbc_in_dev write the interior planes (num_eqn, my, mx) of
+ * q_out_dev.  The x call folds its speed maximum into ctrl_dev and leaves the step open; the y
+ * call closes it: ctrl_dev->cfl is then the maximum of both sweeps' CFL numbers and the status
+ * bits are those of the stored q^{n+1}.  Call them in pairs, in that order, on one stream.  Other
+ * arguments as claw2d_step_padded (-27: ids not 1 or 2).  Replaces classic2.step2ds(..., ids)
+ * called from ClawSolver2D.step_hyperbolic with ghost cells refilled between the two directions
+ * (upstream classic/solver.py, classic/step2ds.f90). */
+int claw2d_sweep_padded(const claw2d_desc* d, const void* qbc_in_dev, void* q_out_dev,
+                        const void* aux_dev, double dt, int32_t ids, claw2d_ctrl* ctrl_dev,
+                        void* stream);
 
 /* Bytes of the workspace claw2d_mass_sum needs. */
 size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc* d);

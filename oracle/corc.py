@@ -331,7 +331,8 @@ def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, n
     an accepted step source(q, dt) (Godunov, source_split 1) or source(q, dt/2) (Strang); a
     rejected attempt restores q and t.  source and before_step act on q (num_eqn, mx, my) in
     place.  With custom BC codes (0) in c the ghost cells come from apply_bcs with the user
-    functions and the step is orc_step2 on that qbc (unsplit).  The accept/reject, t and dt rules are orc_evolve's; nsteps stops after that many
+    functions and the step is orc_step2 on that qbc, or with dimensional splitting the x-sweeps of
+    orc_step2ds, apply_bcs on q* and the y-sweeps.  The accept/reject, t and dt rules are orc_evolve's; nsteps stops after that many
     accepted steps (no clipping to tend), tend runs to that time.  The hyperbolic step is
     orc_step_hyperbolic."""
     q = np.array(q, dtype=np.float64, order="F", copy=True)
@@ -354,7 +355,14 @@ def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, n
             g = d.num_ghost
             auxbc = None if aux is None else fill_aux_ghosts(d, c, aux)
             qbc = apply_bcs(d, c, q, t, user_bc_lower, user_bc_upper, auxbc=auxbc)
-            qnbc, cfl = step2(d, qbc, dt, auxbc=auxbc)
+            if c.dimensional_split:
+                # orc_step_hyperbolic_aux's order: x-sweeps, the ghost fill of q*, y-sweeps
+                qsbc, cx = step2ds(d, qbc, dt, 1, auxbc=auxbc)
+                qsbc = apply_bcs(d, c, qsbc[:, g:-g, g:-g], t, user_bc_lower, user_bc_upper, auxbc=auxbc)
+                qnbc, cy = step2ds(d, qsbc, dt, 2, auxbc=auxbc)
+                cfl = max(cx, cy)
+            else:
+                qnbc, cfl = step2(d, qbc, dt, auxbc=auxbc)
             qn = np.asfortranarray(qnbc[:, g:-g, g:-g])
         else:
             qn, cfl = step_hyperbolic(d, c, q, dt, aux=aux)

@@ -161,3 +161,24 @@ def test_step_padded_accepts_custom_codes_and_checks_arguments():
     # aux boundary conditions are still index maps: custom is refused for them
     aux_custom = _desc(solver_id=2, num_aux=1, fwave=1, bc_lower=(0, 3), aux_bc_lower=(0, 3), aux_bc_upper=(1, 1))
     assert lib.claw2d_step_padded(C.byref(aux_custom), qa, qb, C.c_void_p(16384), 0.1, ctrl, None) == -19
+
+
+def test_sweep_padded_checks_arguments():
+    """claw2d_sweep_padded: the padded descriptor rules of claw2d_step_padded plus the sweep
+    direction (1 or 2); every error is refused before any device call."""
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    d = _desc(bc_lower=(0, 1), bc_upper=(1, 0))
+    qa, qb, ctrl = C.c_void_p(4096), C.c_void_p(8192), C.c_void_p(12288)
+    for ids in (0, 3, -1):
+        assert lib.claw2d_sweep_padded(C.byref(d), qa, qb, None, 0.1, ids, ctrl, None) == -27
+    for ids in (1, 2):
+        assert lib.claw2d_sweep_padded(C.byref(d), None, qb, None, 0.1, ids, ctrl, None) == -22
+        assert lib.claw2d_sweep_padded(C.byref(d), qa, qb, None, 0.1, ids, None, None) == -22
+        assert lib.claw2d_sweep_padded(C.byref(d), qa, qa, None, 0.1, ids, ctrl, None) == -23
+        assert lib.claw2d_sweep_padded(C.byref(d), qa, qb, None, -1.0, ids, ctrl, None) == -21
+    assert lib.claw2d_sweep_padded(None, qa, qb, None, 0.1, 1, ctrl, None) == -1
+    bad = _desc(bc_lower=(7, 1))
+    assert lib.claw2d_sweep_padded(C.byref(bad), qa, qb, None, 0.1, 1, ctrl, None) == -10
+    aux_d = _desc(solver_id=2, num_aux=1, fwave=1, bc_lower=(0, 3), aux_bc_lower=(1, 3), aux_bc_upper=(1, 1))
+    assert lib.claw2d_sweep_padded(C.byref(aux_d), qa, qb, None, 0.1, 2, ctrl, None) == -22

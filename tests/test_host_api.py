@@ -65,10 +65,9 @@ def test_bc_checks():
     with pytest.raises(Exception, match="Lower custom BC function has not been set"):
         solver.setup(sol)
     solver.user_bc_lower = lambda state, dim, t, qbc, auxbc, num_ghost: None
-    solver.dimensional_split = True
-    with pytest.raises(NotImplementedError, match="dimensional_split"):
-        solver.setup(sol)
-    solver.dimensional_split = False
+    for split in (True, False):  # custom sides run with both step forms
+        solver.dimensional_split = split
+        solver._check_supported(state)
     solver.bc_lower = [pyclaw.BC.extrap, pyclaw.BC.extrap]
     solver.bc_upper = [pyclaw.BC.extrap, pyclaw.BC.custom]
     with pytest.raises(Exception, match="Upper custom BC function has not been set"):

@@ -106,18 +106,47 @@ def extrap_user(side):
     return fn
 
 
-def test_custom_function_reproducing_extrap_gives_the_extrap_run():
+@pytest.mark.parametrize("split", [False, True])
+def test_custom_function_reproducing_extrap_gives_the_extrap_run(split):
+    """With splitting the user function also fills q*'s ghost cells between the sweeps, as the
+    standard rules are refilled there, so the runs agree bit for bit either way."""
     q, dx, dy = problems.radial_dam_break(40, 30)
     d = corc.make_desc("swe", 40, 30, dx, dy, grav=1.0)
-    ref, sr = corc.evolve(d, corc.make_ctrl(bc_lower=(1, 3), bc_upper=(1, 3)), q, tend=0.4)
-    c = corc.make_ctrl(bc_lower=(0, 3), bc_upper=(0, 3))
+    ref, sr = corc.evolve(d, corc.make_ctrl(bc_lower=(1, 3), bc_upper=(1, 3), dimensional_split=split),
+                          q, tend=0.4)
+    c = corc.make_ctrl(bc_lower=(0, 3), bc_upper=(0, 3), dimensional_split=split)
     got, sg = corc.evolve_with_source(d, c, q, tend=0.4, user_bc_lower=extrap_user(0),
                                       user_bc_upper=extrap_user(1))
     assert sg["numsteps"] == sr["numsteps"] and sg["t"] == sr["t"]
     assert np.array_equal(got, ref)
 
 
-def test_custom_inflow_boundary_sends_a_bore():
+def test_split_calls_the_user_function_before_each_sweep():
+    """Split step with a custom lower-x side: two calls per attempt (q^n, then q*), both for
+    dimension 0 at the attempt's t, the second seeing the x-swept interior."""
+    q, dx, dy = problems.radial_dam_break(24, 20)
+    d = corc.make_desc("swe", 24, 20, dx, dy, grav=1.0)
+    c = corc.make_ctrl(bc_lower=(0, 1), bc_upper=(1, 1), dimensional_split=True)
+    calls = []
+    ext = extrap_user(0)
+
+    def fn(state, idim, t, qbc, auxbc, g):
+        calls.append((idim, t, qbc[:, g:-g, g:-g].copy()))
+        ext(state, idim, t, qbc, auxbc, g)
+
+    dt = 0.01
+    qn, st = corc.evolve_with_source(d, c, q, nsteps=3, dt=dt, user_bc_lower=fn)
+    assert st["rejected"] == 0 and len(calls) == 2 * st["attempts"] == 6
+    assert all(idim == 0 for idim, _, _ in calls)
+    assert calls[0][1] == calls[1][1] == 0.0
+    assert np.array_equal(calls[0][2], q)
+    qbc = corc.apply_bcs(d, corc.make_ctrl(bc_lower=(1, 1), bc_upper=(1, 1)), q)
+    qs, _ = corc.step2ds(d, qbc, dt, 1)
+    assert np.array_equal(calls[1][2], qs[:, 2:-2, 2:-2])
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_custom_inflow_boundary_sends_a_bore(split):
     """An inflow state h = 1.5, hu = 0.6 imposed in the left ghost columns of still water h = 1
     drives a bore into the domain: mass grows by about the inflow rate times t, and the far right
     stays at rest."""
@@ -125,7 +154,7 @@ def test_custom_inflow_boundary_sends_a_bore():
     q = np.zeros((3, mx, my), order="F")
     q[0] = 1.0
     d = corc.make_desc("swe", mx, my, 1.0 / mx, 1.0 / my, grav=1.0)
-    c = corc.make_ctrl(bc_lower=(0, 3), bc_upper=(1, 3))
+    c = corc.make_ctrl(bc_lower=(0, 3), bc_upper=(1, 3), dimensional_split=split)
 
     def inflow(state, idim, t, qbc, auxbc, g):
         qbc[0, :g, :] = 1.5
