@@ -50,7 +50,7 @@ int set_error(int code, const char* msg) { return fail(code, msg); }
 int set_hip_error(hipError_t e, const char* where) { return hip_fail(e, where); }
 
 // ------------------------------------------------------------------------------------------
-// cross-lane helpers (LDS crossbar, not VALU)
+// cross-lane helpers: ds_bpermute (LDS crossbar) and DPP wavefront shifts (VALU)
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ double shfl(double v, int addr) {
   const long long b = __double_as_longlong(v);
@@ -61,6 +61,46 @@ __device__ __forceinline__ double shfl(double v, int addr) {
 __device__ __forceinline__ float shfl(float v, int addr) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
 }
+
+// Neighbour exchange within a wavefront: lane i receives lane i-1's value (from_left) or lane
+// i+1's (from_right); lane 0 (from_left) and lane 63 (from_right) keep their own value.  DPP:
+// wavefront shifts (wave_shr:1 / wave_shl:1; a source lane out of range keeps the old value), VALU
+// moves with no LDS round trip; otherwise ds_bpermute.  Every call site is in wave-uniform control
+// flow (all 64 lanes active), where both forms give the same values.
+template <int CTRL> __device__ __forceinline__ int dpp_shift(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ double dpp_shift(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_shift<CTRL>((int)(b & 0xffffffffLL));
+  const int hi = dpp_shift<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_shift(float v) {
+  return __int_as_float(dpp_shift<CTRL>(__float_as_int(v)));
+}
+constexpr int kDppWaveShr1 = 0x138, kDppWaveShl1 = 0x130;
+template <bool DPP, class T> __device__ __forceinline__ T from_left(T v) {
+  if constexpr (DPP) {
+    return dpp_shift<kDppWaveShr1>(v);
+  } else {
+    const int lane = threadIdx.x & 63;
+    return shfl(v, (lane > 0 ? lane - 1 : 0) * 4);
+  }
+}
+template <bool DPP, class T> __device__ __forceinline__ T from_right(T v) {
+  if constexpr (DPP) {
+    return dpp_shift<kDppWaveShl1>(v);
+  } else {
+    const int lane = threadIdx.x & 63;
+    return shfl(v, (lane < 63 ? lane + 1 : 63) * 4);
+  }
+}
+// Which form a step kernel uses (measured on MI355X): DPP for FP64 and for the Euler policy, where
+// at 2-3 wavefronts per SIMD the ds_bpermute latency is on the critical path (FP64 shallow water
+// -4%, Euler FP64 -9%, Euler FP32 -7%); ds_bpermute for FP32 shallow water, which at 4 wavefronts
+// per SIMD hides that latency and is bound by VALU issue, where the DPP moves cost 8%.
+template <class T, class P> constexpr bool kDppShift = sizeof(T) == 8 || P::NEQ == 4;
 
 // max that lets a NaN through (so a NaN speed poisons the CFL number instead of vanishing)
 template <class T> __device__ __forceinline__ T nmax(T acc, T s) { return (s > acc || s != s) ? s : acc; }
@@ -231,6 +271,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
                                            const int x0, const int y0, const int y1, const double dt,
                                            const double dx, const double dy, claw2d_ctrl* ctrl) {
   static_assert(SWEEP == 0 || (SPLIT && PAD), "single sweeps run on the padded split path");
+  constexpr bool DPP = kDppShift<T, P>;
   constexpr int NEQ = P::NEQ, NW = P::NW, NC = P::NC, NL = P::NL;
   // 4 equations: add each y-interface's upper-cell shares at once (two more exchanges per row)
   // instead of carrying them to the next row, so the kernel fits 2 wavefronts per SIMD
@@ -249,8 +290,6 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
     bool fa;
     map_index(c, mx, a.abx_lo, a.abx_hi, cxa, fa);
   }
-  const int addrL = (lane > 0 ? lane - 1 : 0) * 4;
-  const int addrR = (lane < 63 ? lane + 1 : 63) * 4;
   const bool own_col = lane >= 2 && lane < 2 + kOwn && c < mx;
   const bool xcfl_col = lane >= 1 && c >= 0 && c <= mx;   // x-interfaces 1..mx+1 (1-based)
   const bool ycfl_col = c >= -1 && c <= mx;               // y-sweeps on columns 0..mx+1
@@ -327,7 +366,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       T pk[NC];
       P::pack(cR, pk);
 #pragma unroll
-      for (int k = 0; k < NC; ++k) pl[k] = shfl(pk[k], addrL);
+      for (int k = 0; k < NC; ++k) pl[k] = from_left<DPP>(pk[k]);
     }
 
     // ---- y-sweep part of row R: the y-interface between rows R-1 and R (in-lane), then the
@@ -361,8 +400,8 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
             // both cells' shares go into their accumulators now (rows R-2 and R-1)
   #pragma unroll
             for (int m = 0; m < NEQ; ++m) {
-              const T l2 = shfl(LG2[m], addrL), l1 = shfl(LG1[m], addrR);
-              const T r2 = shfl(RG2[m], addrL), r1 = shfl(RG1[m], addrR);
+              const T l2 = from_left<DPP>(LG2[m]), l1 = from_right<DPP>(LG1[m]);
+              const T r2 = from_left<DPP>(RG2[m]), r1 = from_right<DPP>(RG1[m]);
               acc0[m] += ((LX[m] + LG1[m]) - LG2[m]) + (l2 - l1);
               acc1[m] += ((RX[m] + RG1[m]) - RG2[m]) + (r2 - r1);
             }
@@ -375,8 +414,8 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
                 const T Y = yX[m] + LX[m];
                 const T G1 = yG1[m] + LG1[m];
                 const T G2 = yG2[m] + LG2[m];
-                const T g2l = shfl(G2, addrL);
-                const T g1r = shfl(G1, addrR);
+                const T g2l = from_left<DPP>(G2);
+                const T g1r = from_right<DPP>(G1);
                 acc0[m] += ((Y + G1) - G2) + (g2l - g1r);
               }
             }
@@ -426,8 +465,8 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       P::lim(fxi, lself);
 #pragma unroll
       for (int k = 0; k < NL; ++k) {
-        lnb[k] = shfl(lself[k], addrR);
-        lleft[k] = shfl(lself[k], addrL);
+        lnb[k] = from_right<DPP>(lself[k]);
+        lleft[k] = from_left<DPP>(lself[k]);
       }
       P::mdot(lself, lnb, mR);
       P::mdot(lleft, lself, mL);
@@ -442,7 +481,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         // q* of row R: the x-sweep only (no transverse terms); valid on the owned columns
         T qs[NEQ];
 #pragma unroll
-        for (int m = 0; m < NEQ; ++m) qs[m] = qR[m] + (RX[m] + shfl(LX[m], addrR));
+        for (int m = 0; m < NEQ; ++m) qs[m] = qR[m] + (RX[m] + from_right<DPP>(LX[m]));
         if constexpr (SWEEP == 1) {
           if (own_col) {
             const size_t off = (size_t)(R + 2) * (size_t)(mx + 4) + (size_t)(c + 2);
@@ -455,9 +494,9 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       } else {
 #pragma unroll
         for (int m = 0; m < NEQ; ++m) {
-          const T X = RX[m] + shfl(LX[m], addrR);
-          const T G1 = RG1[m] + shfl(LG1[m], addrR);
-          const T G2 = RG2[m] + shfl(LG2[m], addrR);
+          const T X = RX[m] + from_right<DPP>(LX[m]);
+          const T G1 = RG1[m] + from_right<DPP>(LG1[m]);
+          const T G2 = RG2[m] + from_right<DPP>(LG2[m]);
           acc1[m] -= G1;
           acc2[m] += (X + G1) - G2;
           acc3[m] = G2;  // row R+1 has no other contribution yet
