@@ -86,6 +86,28 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": n * 8, "repeats": 5}), flush=True)
 
 
+def measured_copy_gbs(nbytes=1 << 31, reps=5):
+    """Device-to-device copy bandwidth of this GPU (read + write bytes per second, median of reps
+    copies of nbytes), the measured second denominator of the HBM roofline."""
+    import numpy as np
+    import torch
+    n = nbytes // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    s = torch.cuda.current_stream()
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        dst.copy_(src)
+        e1.record(s)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    del src, dst
+    return 2.0 * n * 8 / (float(np.median(times)) * 1e-3) / 1e9
+
+
 def run_gpu(n, dtype_id, steps, warmup, seed, split=False):
     import torch
     from clawpack_amd import problems
@@ -390,8 +412,8 @@ def host_threads():
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py measures one GPU: the multi-GPU domain decomposition is not built in this "
-              "round (every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: a multi-GPU domain decomposition is not implemented; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     torch.cuda.init()
@@ -426,6 +448,9 @@ def main():
                      "kernel": "step2_fused<double,SweP<double>,4,false,2> (one launch per attempt)",
                      "bytes_per_cell": bytes_per_cell},
     }
+    copy_gbs = measured_copy_gbs()
+    out["roofline"]["measured_copy_gbs"] = copy_gbs
+    out["roofline"]["frac_of_measured_copy"] = achieved / copy_gbs
     if not a.no_fp32:
         r32 = run_gpu(a.n, 1, a.steps, a.warmup, a.seed)
         v32 = r32["cells"] * r32["accepted"] / (r32["ms"] * 1e-3)
