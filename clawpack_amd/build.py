@@ -13,9 +13,6 @@ FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
     # packed FP32 VALU forms are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # LLVM's ILP-first machine scheduler: the step kernels run at 2-4 wavefronts per SIMD, where
-    # latency hiding within a wavefront pays (0.5-2.6% faster per kernel on MI355X, same registers)
-    "-mllvm", "-amdgpu-sched-strategy=iterative-ilp",
     "-Wall", "-Wno-unused-function",
 ]
 

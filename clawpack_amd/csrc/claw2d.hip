@@ -225,14 +225,16 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
     P::tbeta(f, B, bB);
     const T w = T(-0.5) * r * rt;
     // transversely subsonic on every lane (v - a < 0 < v + a): the down/up split has a fixed
-    // pattern, so the zero entries are skipped; the result is the same bit for bit
-    const bool sub = f.v - f.a < T(0) && f.v + f.a > T(0);
+    // pattern, so the zero entries are skipped; the result is the same bit for bit, because both
+    // forms take their interface constants from the one tc formed here
+    const typename P::TConst tc = P::tconst(f);
+    const bool sub = tc.vm < T(0) && tc.vp > T(0);
     if (__ballot(!sub) == 0ull) {
-      P::expand_t_split(f, w, bA, LG1, LG2);
-      P::expand_t_split(f, w, bB, RG1, RG2);
+      P::expand_t_split(f, tc, w, bA, LG1, LG2);
+      P::expand_t_split(f, tc, w, bB, RG1, RG2);
     } else {
       T sg[NW], c1[NW], c2[NW], c3[NW], c4[NW];
-      P::tspeeds(f, sg);
+      P::tspeeds(f, tc, sg);
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
         const T wm = w * vmin(sg[j], T(0)), wp = w * vmax(sg[j], T(0));
@@ -241,10 +243,10 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
         c3[j] = wm * bB[j];
         c4[j] = wp * bB[j];
       }
-      P::expand_t(f, c1, LG1);
-      P::expand_t(f, c2, LG2);
-      P::expand_t(f, c3, RG1);
-      P::expand_t(f, c4, RG2);
+      P::expand_t(f, tc, c1, LG1);
+      P::expand_t(f, tc, c2, LG2);
+      P::expand_t(f, tc, c3, RG1);
+      P::expand_t(f, tc, c4, RG2);
     }
   }
 }

@@ -210,29 +210,33 @@ template <class T> struct SweP {
     b[1] = f.a * (A[2] - A[0]);
     b[2] = m + d;
   }
-  __device__ __forceinline__ static void tspeeds(const Iface& f, T sg[NW]) {
-    sg[0] = f.v - f.a; sg[1] = f.v; sg[2] = f.v + f.a;
+  // Interface constants of the transverse eigenvectors, formed once per interface and shared by
+  // both forms of the transverse split, so the two forms see the same bits.
+  struct TConst { T vm, vp; };  // v - a, v + a
+  __device__ __forceinline__ static TConst tconst(const Iface& f) { return TConst{f.v - f.a, f.v + f.a}; }
+  __device__ __forceinline__ static void tspeeds(const Iface& f, const TConst& tc, T sg[NW]) {
+    sg[0] = tc.vm; sg[1] = f.v; sg[2] = tc.vp;
   }
   // sum_j c_j f_j
-  __device__ __forceinline__ static void expand_t(const Iface& f, const T c[NW], T o[NEQ]) {
+  __device__ __forceinline__ static void expand_t(const Iface& f, const TConst& tc, const T c[NW], T o[NEQ]) {
     o[0] = c[0] + c[2];
     o[1] = fma(o[0], f.u, c[1]);
-    o[2] = fma(c[2], f.v + f.a, c[0] * (f.v - f.a));
+    o[2] = fma(c[2], tc.vp, c[0] * tc.vm);
   }
   // expand_t of the down-going (wm) and up-going (wp) halves of w*beta when v - a < 0 < v + a:
   // wave 1 is down-going, wave 3 up-going, the shear wave goes with the sign of v.  Bitwise equal
   // to expand_t of the general split, whose other entries are (signed) zeros.
-  __device__ __forceinline__ static void expand_t_split(const Iface& f, T w, const T b[NW], T Gm[NEQ],
-                                                        T Gp[NEQ]) {
-    const T c0 = (w * (f.v - f.a)) * b[0];
+  __device__ __forceinline__ static void expand_t_split(const Iface& f, const TConst& tc, T w, const T b[NW],
+                                                        T Gm[NEQ], T Gp[NEQ]) {
+    const T c0 = (w * tc.vm) * b[0];
     const T c1m = (w * vmin(f.v, T(0))) * b[1], c1p = (w * vmax(f.v, T(0))) * b[1];
-    const T c2 = (w * (f.v + f.a)) * b[2];
+    const T c2 = (w * tc.vp) * b[2];
     Gm[0] = c0;
     Gm[1] = fma(c0, f.u, c1m);
-    Gm[2] = c0 * (f.v - f.a);
+    Gm[2] = c0 * tc.vm;
     Gp[0] = c2;
     Gp[1] = fma(c2, f.u, c1p);
-    Gp[2] = c2 * (f.v + f.a);
+    Gp[2] = c2 * tc.vp;
   }
 };
 
@@ -388,34 +392,54 @@ template <class T> struct EulerP {
     b[2] = f.a * (A[3] - A[0]);
     b[3] = m + d;
   }
-  __device__ __forceinline__ static void tspeeds(const Iface& f, T sg[NW]) {
-    sg[0] = f.v - f.a; sg[1] = f.v; sg[2] = f.v; sg[3] = f.v + f.a;
+  // Interface constants of the transverse eigenvectors, formed once per interface and shared by
+  // both forms of the transverse split.  Each form used to evaluate H -+ v a and (u^2 + v^2)/2
+  // itself, and the compiler may contract such an expression differently in the two copies; the
+  // form is chosen per wavefront, so the rounding then depended on a column's position in its strip.
+  struct TConst { T vm, vp, k, Hm, Hp; };  // v - a, v + a, (u^2 + v^2)/2, H - v a, H + v a
+  __device__ __forceinline__ static TConst tconst(const Iface& f) {
+    TConst t;
+    t.vm = f.v - f.a;
+    t.vp = f.v + f.a;
+    t.k = T(0.5) * (f.u * f.u + f.v * f.v);
+    t.Hm = f.H - f.v * f.a;
+    t.Hp = f.H + f.v * f.a;
+    return t;
   }
-  __device__ __forceinline__ static void expand_t(const Iface& f, const T c[NW], T o[NEQ]) {
-    const T k = T(0.5) * (f.u * f.u + f.v * f.v);
-    const T sum = (c[0] + c[1]) + c[3];
-    o[0] = sum;
-    o[1] = fma(sum, f.u, c[2]);
-    o[2] = fma(c[3], f.v + f.a, fma(c[1], f.v, c[0] * (f.v - f.a)));
-    o[3] = fma(c[3], f.H + f.v * f.a, fma(c[2], f.u, fma(c[1], k, c[0] * (f.H - f.v * f.a))));
+  __device__ __forceinline__ static void tspeeds(const Iface& f, const TConst& tc, T sg[NW]) {
+    sg[0] = tc.vm; sg[1] = f.v; sg[2] = f.v; sg[3] = tc.vp;
+  }
+  // Both forms below are written with explicit fma() and contraction off, so that they are the same
+  // IEEE operations on every lane (a contracted "a*b + c" could be fused differently in the two).
+  __device__ __forceinline__ static void expand_t(const Iface& f, const TConst& tc, const T c[NW], T o[NEQ]) {
+    {
+#pragma clang fp contract(off)
+      const T sum = (c[0] + c[1]) + c[3];
+      o[0] = sum;
+      o[1] = fma(sum, f.u, c[2]);
+      o[2] = fma(c[3], tc.vp, fma(c[1], f.v, c[0] * tc.vm));
+      o[3] = fma(c[3], tc.Hp, fma(c[2], f.u, fma(c[1], tc.k, c[0] * tc.Hm)));
+    }
   }
   // expand_t of the down-going and up-going halves of w*beta when v - a < 0 < v + a (wave 1
   // down, wave 4 up, entropy and shear waves with the sign of v); bitwise equal to expand_t of the
   // general split, whose other entries are (signed) zeros.
-  __device__ __forceinline__ static void expand_t_split(const Iface& f, T w, const T b[NW], T Gm[NEQ],
-                                                        T Gp[NEQ]) {
-    const T k = T(0.5) * (f.u * f.u + f.v * f.v);
-    const T wvm = w * vmin(f.v, T(0)), wvp = w * vmax(f.v, T(0));
-    const T c0 = (w * (f.v - f.a)) * b[0], c3 = (w * (f.v + f.a)) * b[3];
-    const T c1m = wvm * b[1], c2m = wvm * b[2], c1p = wvp * b[1], c2p = wvp * b[2];
-    Gm[0] = c0 + c1m;
-    Gm[1] = fma(Gm[0], f.u, c2m);
-    Gm[2] = fma(c1m, f.v, c0 * (f.v - f.a));
-    Gm[3] = fma(c2m, f.u, fma(c1m, k, c0 * (f.H - f.v * f.a)));
-    Gp[0] = c1p + c3;
-    Gp[1] = fma(Gp[0], f.u, c2p);
-    Gp[2] = fma(c3, f.v + f.a, c1p * f.v);
-    Gp[3] = fma(c3, f.H + f.v * f.a, fma(c2p, f.u, c1p * k));
+  __device__ __forceinline__ static void expand_t_split(const Iface& f, const TConst& tc, T w, const T b[NW],
+                                                        T Gm[NEQ], T Gp[NEQ]) {
+    {
+#pragma clang fp contract(off)
+      const T wvm = w * vmin(f.v, T(0)), wvp = w * vmax(f.v, T(0));
+      const T c0 = (w * tc.vm) * b[0], c3 = (w * tc.vp) * b[3];
+      const T c1m = wvm * b[1], c2m = wvm * b[2], c1p = wvp * b[1], c2p = wvp * b[2];
+      Gm[0] = c0 + c1m;
+      Gm[1] = fma(Gm[0], f.u, c2m);
+      Gm[2] = fma(c1m, f.v, c0 * tc.vm);
+      Gm[3] = fma(c2m, f.u, fma(c1m, tc.k, c0 * tc.Hm));
+      Gp[0] = c1p + c3;
+      Gp[1] = fma(Gp[0], f.u, c2p);
+      Gp[2] = fma(c3, tc.vp, c1p * f.v);
+      Gp[3] = fma(c3, tc.Hp, fma(c2p, f.u, c1p * tc.k));
+    }
   }
 };
 

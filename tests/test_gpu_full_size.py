@@ -1,0 +1,223 @@
+"""Parity at the full benchmark sizes, through checks that do not depend on size.
+
+  * a few controller-driven steps at 4096^2 and 8192^2 (shallow water, periodic, MC, the seeded
+    hump field) and 2048^2 (Euler, the four-quadrant problem) against the multi-threaded oracle,
+    to 1e-12 * max|q|, with the same accepted/rejected counts and t;
+  * translation invariance, bit for bit: with periodic boundaries a state rolled by (1001, 37)
+    cells gives the rolled result.  Every cell runs the same arithmetic wherever the 60-column
+    strips and the row segments fall, so a halo, wrap or segment-edge mistake breaks it;
+  * conservation of mass and momentum with periodic boundaries at 8192^2 (compensated FP64 sums);
+  * a uniform state is a fixed point, bit for bit;
+  * the 512-grid batch: one batched launch per attempt equals one-grid-at-a-time launches, bit
+    for bit, for every grid.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from clawpack_amd import _lib, problems
+from clawpack_amd.device import DeviceGrid, make_desc
+from oracle import corc
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+SHIFT = (1001, 37)  # (rows, columns): crosses strip and segment boundaries
+
+
+def oracle_threads():
+    corc.set_threads(max(1, min(16, os.cpu_count() or 1)))
+
+
+def humps_planes(n, seed=2024, n_humps=16):
+    """The seeded hump field of the shallow-water workload (h = 1 + sum_k A_k exp(-|x-c_k|^2 /
+    (2 w_k^2)) on [-2.5, 2.5]^2, hu = hv = 0), evaluated in separable form with torch on the host:
+    planes (3, n, n) float64, row index y."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.05, 0.5, n_humps)
+    W = rng.uniform(0.02, 0.2, n_humps)
+    cx = rng.uniform(-2.5, 2.5, n_humps)
+    cy = rng.uniform(-2.5, 2.5, n_humps)
+    xc, dx = problems.centers(-2.5, 2.5, n)
+    x = torch.from_numpy(xc)
+    h = torch.ones((n, n), dtype=torch.float64)
+    for k in range(n_humps):
+        ex = torch.exp(-(x - cx[k]) ** 2 / (2.0 * W[k] ** 2))
+        ey = torch.exp(-(x - cy[k]) ** 2 / (2.0 * W[k] ** 2))
+        h.add_(torch.outer(ey, ex), alpha=float(A[k]))
+    planes = torch.zeros((3, n, n), dtype=torch.float64)
+    planes[0] = h
+    return planes, dx
+
+
+def to_pyclaw(planes):
+    return np.asfortranarray(planes.permute(0, 2, 1).numpy())
+
+
+def swe_desc(n, dx, dtype=_lib.FP64, bc=2):
+    return make_desc(0, dtype, n, n, dx, dx, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
+                     bc_lower=(bc, bc), bc_upper=(bc, bc), grav=1.0)
+
+
+def euler_desc(n, dx, dy, dtype=_lib.FP64, bc=1):
+    return make_desc(1, dtype, n, n, dx, dy, 4, 4, order=2, transverse_waves=2, mthlim=(4, 4, 4, 4),
+                     bc_lower=(bc, bc), bc_upper=(bc, bc), gamma=1.4)
+
+
+def euler_dt(q, dx, dy, gamma=1.4, cfl=0.8):
+    rho = q[0]
+    u, v = q[1] / rho, q[2] / rho
+    p = (gamma - 1.0) * (q[3] - 0.5 * rho * (u * u + v * v))
+    c = np.sqrt(gamma * p / rho)
+    return cfl / max(float((np.abs(u) + c).max()) / dx, float((np.abs(v) + c).max()) / dy)
+
+
+@pytest.mark.parametrize("n,split", [(4096, False), (4096, True), (8192, False)])
+def test_shallow_water_full_size_matches_oracle(n, split):
+    print("case full-size swe vs oracle n=%d split=%s" % (n, split), flush=True)
+    planes, dx = humps_planes(n)
+    dt0 = 0.8 * dx / math.sqrt(float(planes[0].max()))
+    g = DeviceGrid(swe_desc(n, dx), DEV, split=split)
+    g.set_planes(planes.to(DEV))
+    g.init_ctrl(0.0, dt0, target_steps=2)
+    g.advance(4)
+    c = g.sync_cur()
+    qg = g.download()
+    del g
+    print("  device done", flush=True)
+    q = to_pyclaw(planes)
+    del planes
+    oracle_threads()
+    od = corc.make_desc("swe", n, n, dx, dx, grav=1.0)
+    oc = corc.make_ctrl(bc_lower=(2, 2), bc_upper=(2, 2), dimensional_split=split)
+    qo, st = corc.evolve(od, oc, q, nsteps=2, dt=dt0)
+    print("  oracle done", flush=True)
+    assert c.numsteps == st["numsteps"] == 2 and c.rejected == st["rejected"]
+    assert abs(c.t - st["t"]) <= 1e-12 * st["t"]
+    err = float(np.abs(qg - qo).max()) / float(np.abs(q).max())
+    assert err <= 1e-12, err
+
+
+def test_euler_2048_matches_oracle():
+    print("case full-size euler vs oracle", flush=True)
+    n = 2048
+    q, dx, dy = problems.euler_quadrants(n, n, 1)
+    dt0 = euler_dt(q, dx, dy)
+    g = DeviceGrid(euler_desc(n, dx, dy), DEV)
+    g.upload(q)
+    g.init_ctrl(0.0, dt0, target_steps=3)
+    g.advance(5)
+    c = g.sync_cur()
+    qg = g.download()
+    print("  device done", flush=True)
+    oracle_threads()
+    od = corc.make_desc("euler", n, n, dx, dy, gamma=1.4)
+    qo, st = corc.evolve(od, corc.make_ctrl(), q, nsteps=3, dt=dt0)
+    assert c.numsteps == st["numsteps"] == 3 and c.rejected == st["rejected"]
+    assert abs(c.t - st["t"]) <= 1e-12 * st["t"]
+    err = float(np.abs(qg - qo).max()) / float(np.abs(q).max())
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("case", ["swe_fp64", "swe_fp32", "swe_split_fp64", "euler_fp64", "euler_fp32"])
+def test_translation_invariance_bitwise(case):
+    print("case translation invariance", case, flush=True)
+    dtype = _lib.FP32 if case.endswith("fp32") else _lib.FP64
+    if case.startswith("swe"):
+        n = 4096
+        planes, dx = humps_planes(n)
+        # a moving field: momentum makes the x and y sweeps differ and both entropy-fix sides occur
+        planes[1] = 0.6 * (planes[0] - 1.0) - 0.05
+        planes[2] = -0.4 * (planes[0] - 1.0) + 0.05
+        d = swe_desc(n, dx, dtype)
+        dt0 = 0.5 * dx / (math.sqrt(float(planes[0].max())) + 0.5)
+    else:
+        n = 2048
+        q, dx, dy = problems.euler_quadrants(n, n, 1)
+        planes = torch.from_numpy(np.ascontiguousarray(q.transpose(0, 2, 1)))
+        d = euler_desc(n, dx, dy, dtype, bc=2)
+        dt0 = euler_dt(q, dx, dy, cfl=0.5)
+    split = "split" in case
+    tdt = torch.float32 if dtype == _lib.FP32 else torch.float64
+    p0 = planes.to(DEV, tdt)
+    grids = []
+    for p in (p0, torch.roll(p0, SHIFT, dims=(1, 2))):
+        g = DeviceGrid(d, DEV, split=split)
+        g.set_planes(p)
+        g.init_ctrl(0.0, dt0, target_steps=20)
+        g.advance(24)
+        grids.append(g)
+    c1, c2 = grids[0].sync_cur(), grids[1].sync_cur()
+    assert c1.status == c2.status == 0
+    assert c1.numsteps == c2.numsteps == 20 and c1.t == c2.t and c1.dt == c2.dt
+    assert torch.equal(torch.roll(grids[0].planes(), SHIFT, dims=(1, 2)), grids[1].planes())
+
+
+@pytest.mark.parametrize("dtype,tol", [(_lib.FP64, 1e-13), (_lib.FP32, 1e-5)])
+def test_conservation_8192_periodic(dtype, tol):
+    print("case conservation 8192 dtype", dtype, flush=True)
+    n = 8192
+    planes, dx = humps_planes(n)
+    planes[1] = 0.3 * (planes[0] - 1.0) + 0.02
+    planes[2] = -0.2 * (planes[0] - 1.0) - 0.01
+    tdt = torch.float32 if dtype == _lib.FP32 else torch.float64
+    g = DeviceGrid(swe_desc(n, dx, dtype), DEV)
+    g.set_planes(planes.to(DEV, tdt))
+    scale = [float(g.planes()[m].double().abs().sum()) for m in range(3)]
+    m0 = [g.mass_sum(m) for m in range(3)]
+    g.init_ctrl(0.0, 0.5 * dx / (math.sqrt(float(planes[0].max())) + 0.5), target_steps=40)
+    g.advance(45)
+    c = g.sync_cur()
+    assert c.numsteps == 40 and c.status == 0
+    m1 = [g.mass_sum(m) for m in range(3)]
+    for m in range(3):
+        assert abs(m1[m] - m0[m]) <= tol * scale[m], (m, m0[m], m1[m])
+
+
+@pytest.mark.parametrize("solver", ["swe", "euler"])
+def test_uniform_state_is_a_fixed_point(solver):
+    print("case uniform fixed point", solver, flush=True)
+    if solver == "swe":
+        n = 8192
+        d = swe_desc(n, 5.0 / n)
+        vals = (1.3, 0.2, -0.1)
+    else:
+        n = 2048
+        d = euler_desc(n, 1.0 / n, 1.0 / n, bc=2)
+        rho, u, v, p = 1.0, 0.3, -0.2, 1.0
+        vals = (rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v))
+    planes = torch.stack([torch.full((n, n), x, dtype=torch.float64, device=DEV) for x in vals])
+    g = DeviceGrid(d, DEV)
+    g.set_planes(planes)
+    g.init_ctrl(0.0, 0.5 / n, target_steps=3)
+    g.advance(4)
+    c = g.sync_cur()
+    assert c.numsteps == 3 and c.status == 0
+    assert torch.equal(g.planes(), planes)
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_batch_512_grids_equals_sequential(dtype):
+    print("case 512-grid batch dtype", dtype, flush=True)
+    from clawpack_amd.batch import BatchedGrids
+    grids = problems.batch_workload(512, 2024)
+    d = make_desc(0, dtype, 8, 8, 1.0, 1.0, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
+                  bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
+    b1, b2 = BatchedGrids(d, grids), BatchedGrids(d, grids)
+    del grids
+    steps = 12
+    for b in (b1, b2):
+        b.init(0.0, 1e-3, target_steps=steps)
+    h = b1.make_graph(steps + 3)
+    b1.replay(h)
+    b2.advance_sequential(steps + 3)
+    torch.cuda.synchronize()
+    print("  advanced", flush=True)
+    c1, c2 = b1.read_ctrls(), b2.read_ctrls()
+    for g in range(b1.n):
+        assert c1[g].status == c2[g].status == 0
+        assert c1[g].numsteps == c2[g].numsteps == steps, g
+        assert c1[g].t == c2[g].t and c1[g].dt == c2[g].dt, g
+        assert np.array_equal(b1.download(g, c1), b2.download(g, c2)), g
