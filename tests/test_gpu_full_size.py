@@ -9,7 +9,7 @@
   * conservation of mass and momentum with periodic boundaries at 8192^2 (compensated FP64 sums);
   * a uniform state is a fixed point, bit for bit;
   * the 512-grid batch: one batched launch per attempt equals one-grid-at-a-time launches, bit
-    for bit, for every grid.
+    for bit, for every grid, and matches the oracle per instance to 1e-12.
 """
 import math
 import os
@@ -221,3 +221,33 @@ def test_batch_512_grids_equals_sequential(dtype):
         assert c1[g].numsteps == c2[g].numsteps == steps, g
         assert c1[g].t == c2[g].t and c1[g].dt == c2[g].dt, g
         assert np.array_equal(b1.download(g, c1), b2.download(g, c2)), g
+
+
+def test_batch_512_grids_match_oracle_per_instance():
+    """The batched workload at full size (512 grids, sides 64..512) against the oracle per grid:
+    three controller-driven steps from dt = 1e-3, each grid with its own dt."""
+    print("case 512-grid batch vs oracle", flush=True)
+    from clawpack_amd.batch import BatchedGrids
+    grids = problems.batch_workload(512, 2024)
+    d = make_desc(0, _lib.FP64, 8, 8, 1.0, 1.0, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
+                  bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
+    b = BatchedGrids(d, grids)
+    steps = 3
+    b.init(0.0, 1e-3, target_steps=steps)
+    b.advance(steps + 4)
+    torch.cuda.synchronize()
+    print("  advanced", flush=True)
+    ctrls = b.read_ctrls()
+    oracle_threads()
+    oc = corc.make_ctrl(bc_lower=(2, 2), bc_upper=(2, 2))
+    worst = 0.0
+    for g, (q, dx, dy) in enumerate(grids):
+        mx, my = q.shape[1:]
+        qo, st = corc.evolve(corc.make_desc("swe", mx, my, dx, dy, grav=1.0), oc, q, nsteps=steps, dt=1e-3)
+        assert ctrls[g].status == 0 and ctrls[g].numsteps == st["numsteps"] == steps, g
+        assert ctrls[g].rejected == st["rejected"], g
+        assert abs(ctrls[g].t - st["t"]) <= 1e-12 * st["t"], g
+        err = float(np.abs(b.download(g, ctrls) - qo).max()) / float(np.abs(q).max())
+        worst = max(worst, err)
+        assert err <= 1e-12, (g, err)
+    print("  worst relative error over 512 grids %.2e" % worst, flush=True)
