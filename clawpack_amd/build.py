@@ -13,6 +13,11 @@ FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
     # packed FP32 VALU forms are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a*b + c is fused into an FMA within one source expression (the front end decides), never
+    # across statements by the backend: backend fusion depends on basic-block boundaries, and the
+    # step kernel's row body exists in two copies with different blocks (steady and edge rows),
+    # which must round alike for results not to depend on a cell's position in its segment
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-function",
 ]
 

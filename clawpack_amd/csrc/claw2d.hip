@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "../../include/clawpack_amd.h"
 #include "physics.hpp"
@@ -352,13 +353,18 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   bool fnext;
   T bnext = T(0);
   load_row(rlo, qnext, fnext, bnext);
-  for (int R = rlo; R <= rhi; ++R) {
+  // One marched row.  ST (steady): a row y0+2..y1-1 of the full step, for which every row test
+  // below holds, so that copy of the body is one straight block the scheduler can interleave
+  // (shallow water: 2.4% faster in FP64 on MI355X; the Euler kernel would spill, so it keeps one
+  // copy of the body).
+  auto row = [&](const int R, auto steady_tag) {
+    constexpr bool ST = decltype(steady_tag)::value;
     T qR[NEQ];
 #pragma unroll
     for (int m = 0; m < NEQ; ++m) qR[m] = qnext[m];
     const T bR = bnext;
     apply_flips(qR, fnext);
-    if (R < rhi) load_row(R + 1, qnext, fnext, bnext);
+    if (ST || R < rhi) load_row(R + 1, qnext, fnext, bnext);
     const Cell cR = prep_cell<P>(qR, bR, a.par);
 
     // ---- request the left neighbour's cell data for the x-interfaces of row R now, so that the
@@ -377,22 +383,22 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
     Iface fynew;
     auto y_part = [&](const Cell& cY) {
       // ---- y-interface R (between rows R-1 and R), in-lane
-      if (R >= y0 - 1) {
+      if (ST || R >= y0 - 1) {
         P::template rpn2<1>(cprev, cY, a.par, fynew);
         // split step: the y-speeds come from q*, which a rejected attempt (too large a dt) can
         // leave with negative depths; as in the reference's max, NaN speeds are skipped (a NaN
         // in an accepted state is still caught by the stored-cell check)
-        if ((SPLIT ? own_col : ycfl_col) && R >= 0 && R <= my)
+        if ((SPLIT ? own_col : ycfl_col) && (ST || (R >= 0 && R <= my)))
           smy = SPLIT ? vmax(smy, P::smax(fynew)) : nmax(smy, P::smax(fynew));
       }
 
       // ---- finish y-interface R-1 (between rows R-2 and R-1)
-      if (R >= y0) {
+      if (ST || R >= y0) {
         T lp[NL], ln[NL], mR[NW];
         P::lim(fyprev, lp);
         P::lim(fynew, ln);
         P::mdot(lp, ln, mR);
-        if (R >= y0 + 1) {
+        if (ST || R >= y0 + 1) {
           T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
           finish_iface<T, P, LIM, TWC>(fyprev, yal2, ymprev, fynew.al, mR, dtdy, dtdx, a, LX, LG1, LG2,
                                   RX, RG1, RG2);
@@ -408,7 +414,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
               acc1[m] += ((RX[m] + RG1[m]) - RG2[m]) + (r2 - r1);
             }
           }
-          if (R >= y0 + 2) {
+          if (ST || R >= y0 + 2) {
             // cell (c, R-2) has all its y-contributions
             if constexpr (!kImmY) {
   #pragma unroll
@@ -454,11 +460,11 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
     // their mapped source rows, which equals refilling the ghosts of q* for extrap, periodic and
     // wall conditions.
     Cell cY = cR;
-    if (SWEEP != 2 && (SPLIT || (R >= y0 - 1 && R <= y1))) {
+    if (SWEEP != 2 && (SPLIT || ST || (R >= y0 - 1 && R <= y1))) {
       const Cell cL = P::unpack(pl);
       Iface fxi;
       P::template rpn2<0>(cL, cR, a.par, fxi);
-      if (SPLIT ? (xcfl_col && R >= 0 && R < my) : (xcfl_col && R >= -1 && R <= my))
+      if (xcfl_col && (ST || (SPLIT ? (R >= 0 && R < my) : (R >= -1 && R <= my))))
         smx = nmax(smx, P::smax(fxi));
       // limiter partners: both neighbours' wave data in one exchange round; the left pair's
       // eigenvector products are recomputed here (the same operands in the same order as on the
@@ -520,6 +526,14 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       cprev = cY;
       fyprev = fynew;
     }
+  };
+  if constexpr (SWEEP == 0 && NEQ == 3) {
+    int R = rlo;
+    for (; R < y0 + 2 && R <= rhi; ++R) row(R, std::false_type{});
+    for (; R <= y1 - 1; ++R) row(R, std::true_type{});
+    for (; R <= rhi; ++R) row(R, std::false_type{});
+  } else {
+    for (int R = rlo; R <= rhi; ++R) row(R, std::false_type{});
   }
 
   // Wave-uniform epilogue: every lane issues the atomics with the same operand (the compiler
