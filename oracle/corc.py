@@ -46,16 +46,17 @@ class Status(C.Structure):
 _lib = None
 
 
-def build():
-    """Compile liborc2d.so from claw2d_oracle.c with gcc (no reference sources involved)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+def build(force=True):
+    """Compile liborc2d.so from claw2d_oracle.c with gcc (no reference sources involved); with
+    force (the default) it recompiles even when the library is newer than the source."""
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE], check=True)
 
 
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB):
-            build()
+            build(force=False)
         L = C.CDLL(_LIB)
         dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
         dpa = C.POINTER(C.c_double)
