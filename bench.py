@@ -144,19 +144,23 @@ def run_gpu(n, dtype_id, steps, warmup, seed, split=False):
 
 def run_batch(ngrids, dtype_id, steps, seq_steps, seed):
     """Batched workload: ngrids independent periodic hump grids (sides U{64..512}), each with its
-    own CFL-0.9 dt; `steps` attempts as one replayed HIP graph vs one-grid-at-a-time launches."""
+    own CFL-0.9 dt; `steps` attempts as one replayed HIP graph vs one-grid-at-a-time launches.
+    The rows per batched work item come from the autotuner, measured here on this workload (64,
+    the library default, is one of the candidates)."""
     import torch
-    from clawpack_amd import problems
+    from clawpack_amd import autotune, problems
     from clawpack_amd.batch import BatchedGrids
     from clawpack_amd.device import make_desc
 
     grids = problems.batch_workload(ngrids, seed)
     d = make_desc(0, dtype_id, 8, 8, 1.0, 1.0, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
                   bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
-    out = {}
+    seg, tune_ms = autotune.tune_batch(d, grids, candidates=(16, 32, 48, 64, 96, 128, 192),
+                                       attempts=5, use_cache=False)
+    out = {"autotune": {"seg_h": seg, "ms_per_attempt": tune_ms}}
     s = torch.cuda.current_stream()
     for mode in ("graph", "sequential"):
-        b = BatchedGrids(d, grids)
+        b = BatchedGrids(d, grids, seg_h=seg)
         b.init(0.0, 1e-3, target_steps=10 ** 9)
         b.advance(3)  # warm-up (first attempts settle each grid's dt)
         n_att = steps if mode == "graph" else seq_steps
