@@ -117,7 +117,8 @@ class ClawSolver2D(Solver):
                 % (self.kernel_language,))
         if self.rp is None or not hasattr(self.rp, "kernel_id"):
             raise NotImplementedError("Riemann solver not available on the HIP path: use "
-                                      "riemann.shallow_roe_with_efix_2D or riemann.euler_4wave_2D")
+                                      "riemann.shallow_roe_with_efix_2D, riemann.euler_4wave_2D or "
+                                      "riemann.shallow_bathymetry_fwave_2D")
         rp_fwave = bool(getattr(self.rp, "fwave", False))
         if bool(self.fwave) != rp_fwave:
             raise ValueError("solver.fwave must be %s for %s (its Riemann solver returns %s)"
