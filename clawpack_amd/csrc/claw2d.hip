@@ -210,7 +210,7 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
       const T aL = alL[p], aR = alR[p], ml = mL[p], mr = mR[p];  // rvalues: no lvalue select
       const T t = (s > T(0) ? aL * ml : aR * mr) * in[p];
       const int meth = LIM > 0 ? LIM : a.mthlim[p];
-      const T sa = s < T(0) ? -s : s;
+      const T sa = vabs(s);  // |s| as a source modifier (a select form costs two instructions)
       const T sw = P::FWAVE ? copysign(T(1), s) : sa;
       k = sw * (T(1) - sa * r) * limited(al, t, meth);
     }

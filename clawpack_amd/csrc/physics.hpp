@@ -173,7 +173,7 @@ template <class T> struct SweP {
     f.u = u; f.v = v; f.a = a; f.ha = ha;
   }
 
-  __device__ __forceinline__ static T smax(const Iface& f) { return (f.u < T(0) ? -f.u : f.u) + f.a; }
+  __device__ __forceinline__ static T smax(const Iface& f) { return vabs(f.u) + f.a; }
   __device__ __forceinline__ static T speed(const Iface& f, int p) {
     return p == 0 ? f.u - f.a : (p == 1 ? f.u : f.u + f.a);
   }
@@ -344,7 +344,7 @@ template <class T> struct EulerP {
     f.u = u; f.v = v; f.a = a; f.H = H; f.i2a = i2a;
   }
 
-  __device__ __forceinline__ static T smax(const Iface& f) { return (f.u < T(0) ? -f.u : f.u) + f.a; }
+  __device__ __forceinline__ static T smax(const Iface& f) { return vabs(f.u) + f.a; }
   __device__ __forceinline__ static T speed(const Iface& f, int p) {
     return p == 0 ? f.u - f.a : (p == 3 ? f.u + f.a : f.u);
   }
