@@ -129,6 +129,13 @@ class FrozenDtEvolve(torch.autograd.Function):
             g_qN.to(torch.float64).contiguous().clone()
         ws = adjoint_workspace(desc, dev)
         gs = None if g_samples is None else g_samples.to(torch.float64)
+        # the reverse steps call the library directly with the constant arguments bound once: the
+        # tensors are this function's own (float64, contiguous, (3, my, mx)), so adjoint_step's
+        # per-call checks would only add host time to a host-bound loop
+        lib = _lib.load()
+        rev = lib.claw2d_adjoint_step
+        dref, wsp, lamp = C.byref(desc), C.c_void_p(ws.data_ptr()), C.c_void_p(lam.data_ptr())
+        sh = _stream_handle(None)
 
         def add_direct(lam, step_index):
             # samples taken after step `step_index` (i.e. of q^{step_index})
@@ -145,7 +152,9 @@ class FrozenDtEvolve(torch.autograd.Function):
                 stepper.step(states[-1], nxt, dts[k])
                 states.append(nxt)
             for k in range(s1 - 1, s0 - 1, -1):
-                adjoint_step(desc, states[k - s0], lam, dts[k], lam_out=lam, workspace=ws)
+                rc = rev(dref, C.c_void_p(states[k - s0].data_ptr()), lamp, lamp, wsp, dts[k], sh)
+                if rc:
+                    check(rc, "claw2d_adjoint_step")
                 add_direct(lam, k)
             del states
         return lam, None, None, None, None, None
