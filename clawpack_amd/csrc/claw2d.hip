@@ -341,6 +341,10 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   };
 
   T smx = T(0), smy = T(0);
+  // FP64: plain max plus a NaN flag folded in once at the end (the same result as a
+  // NaN-propagating max at every update, in two instructions instead of five)
+  constexpr bool kNanFlag = sizeof(T) == 8;
+  bool nanx = false, nany = false;
   bool bad = false;
   Cell cprev = {};
   Iface fyprev;
@@ -398,7 +402,12 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         // leave with negative depths; as in the reference's max, NaN speeds are skipped (a NaN
         // in an accepted state is still caught by the stored-cell check)
         if ((SPLIT ? own_col : ycfl_col) && (ST || (R >= 0 && R <= my)))
-          smy = SPLIT ? vmax(smy, P::smax(fynew)) : nmax(smy, P::smax(fynew));
+        {
+          const T sp = P::smax(fynew);
+          if constexpr (SPLIT) smy = vmax(smy, sp);
+          else if constexpr (kNanFlag) { smy = vmax(smy, sp); nany |= sp != sp; }
+          else smy = nmax(smy, sp);
+        }
       }
 
       // ---- finish y-interface R-1 (between rows R-2 and R-1)
@@ -474,7 +483,11 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       Iface fxi;
       P::template rpn2<0>(cL, cR, a.par, fxi);
       if (xcfl_col && (ST || (SPLIT ? (R >= 0 && R < my) : (R >= -1 && R <= my))))
-        smx = nmax(smx, P::smax(fxi));
+      {
+        const T sp = P::smax(fxi);
+        if constexpr (kNanFlag) { smx = vmax(smx, sp); nanx |= sp != sp; }
+        else smx = nmax(smx, sp);
+      }
       // limiter partners: both neighbours' wave data in one exchange round; the left pair's
       // eigenvector products are recomputed here (the same operands in the same order as on the
       // left lane, so the same bits)
@@ -556,6 +569,10 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   // Wave-uniform epilogue: every lane issues the atomics with the same operand (the compiler
   // folds them into one per wave).  A lane-0-only branch here lets the compiler peel lanes 1-63
   // off into their own loop when this function runs inside the batched kernel's work loop.
+  if constexpr (kNanFlag) {
+    if (nanx) smx = T(NAN);
+    if (nany) smy = T(NAN);
+  }
   const T wx = wave_max(smx), wy = wave_max(smy);
   const bool anybad = __ballot(bad) != 0ull;
   atomic_max_nonneg(&ctrl->smax_bits[0], (double)wx);
