@@ -9,8 +9,8 @@
 //                 right over ds_bpermute; the cell adjoint of column c is the sum of the shares of
 //                 interfaces c and c+1.  Writes the x-sweep share of every extended cell
 //                 (interior + 2 ghost layers) to `ext`, a (my+4) x (mx+4) workspace.
-//   adj_sweep<1>  the same along columns (lane = row position, strided loads), adding the
-//                 y-sweep share into `ext`.
+//   adj_sweep<1>  the same along columns (lane = row position, strided loads; the waves of a
+//                 workgroup take adjacent columns), adding the y-sweep share into `ext`.
 //   adj_fold      lam_out = lam_in + ext(interior) + the ghost-cell shares of ext folded back to
 //                 the interior cells they were copied from (extrap / periodic / wall, with the
 //                 wall's sign), in a fixed order (deterministic, no atomics).
@@ -95,14 +95,26 @@ __device__ __forceinline__ void load_lam(const AdjArgs& a, int x, int y, double 
 template <int DIR>
 __global__ void __launch_bounds__(64 * kWPBA) adj_sweep(AdjArgs a) {
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * kWPBA + (threadIdx.x >> 6));
   const int n_al = DIR == 0 ? a.mx : a.my;  // cells along the sweep line
   const int n_ln = DIR == 0 ? a.my : a.mx;  // sweep lines
   const int nseg = DIR == 0 ? a.nseg_x : a.nseg_y;
+  // DIR 0: the waves of a line's segments are consecutive.  DIR 1 (lanes walk a column, so every
+  // access is strided by the row pitch): the waves of a workgroup take adjacent columns of one
+  // segment, and the workgroups that land on one XCD (dealt to the 8 XCDs in turn) take
+  // consecutive column groups, so the cache lines of neighbouring columns are shared in that
+  // XCD's L2 instead of being fetched by every XCD.  `bb` is a bijection of [0, gridDim.x).
+  // (The same renumbering for DIR 0 measured 9% slower: its accesses are already coalesced.)
+  int bb = blockIdx.x;
+  if (DIR == 1) {
+    const int nb = gridDim.x, q8 = nb >> 3, r8 = nb & 7, x = blockIdx.x & 7, i = blockIdx.x >> 3;
+    bb = x < r8 ? x * (q8 + 1) + i : r8 * (q8 + 1) + (x - r8) * q8 + i;
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(bb * kWPBA + (threadIdx.x >> 6));
   if (wave >= (n_ln + 4) * nseg) return;
-  const int li = wave / nseg;
-  const int line = li - 2;                            // -2 .. n_ln + 1
-  const int p = (wave - li * nseg) * kOwnA - 5 + lane;  // position along: lanes 3..60 own
+  const int li = DIR == 0 ? wave / nseg : wave % (n_ln + 4);
+  const int seg = DIR == 0 ? wave - li * nseg : wave / (n_ln + 4);
+  const int line = li - 2;                  // -2 .. n_ln + 1
+  const int p = seg * kOwnA - 5 + lane;     // position along: lanes 3..60 own
   const bool own = lane >= 3 && lane < 3 + kOwnA && p <= n_al + 1;
   const int ew = a.mx + 4;
   const size_t eplane = (size_t)ew * (size_t)(a.my + 4);
