@@ -858,18 +858,27 @@ static int choose_seg_h(const claw2d_desc* d, long resident) {
   const long nstrips = (d->mx + kOwn - 1) / kOwn;
   long nsegs = resident > 0 ? resident / nstrips : 32;
   long min_rows = 16;
-  const long half = device_simds() / 2;
+  const long simds = device_simds(), half = simds / 2;
   if (nstrips * (((long)d->my + min_rows - 1) / min_rows) < half) {
     nsegs = half / nstrips;
     min_rows = 4;
   }
   nsegs = std::max(1L, std::min(nsegs, ((long)d->my + min_rows - 1) / min_rows));
+  // More than 2 wavefronts per SIMD in segments under 32 rows re-solve more halo rows than the
+  // extra occupancy returns: cut for 2 per SIMD instead.  At 2048^2 (18-row segments at 4 per
+  // SIMD -> 36 rows at 2): shallow water FP32 135.9 -> 114.3 us, split 131.3 -> 107.1 us;
+  // bathymetry FP32 138.1 -> 108.9 us, split 132.4 -> 101.7 us; FP64 split (24 rows at 3 per SIMD)
+  // 128.4 -> 122.5 us.  The Euler kernels keep the occupancy sizing: FP32 135.2 -> 139.2 us there.
+  long seg0 = ((long)d->my + nsegs - 1) / nsegs;
+  long waves = nstrips * (((long)d->my + seg0 - 1) / seg0);
+  if (simds > 0 && d->solver != CLAW_SOLVER_EULER_4WAVE && waves > 2 * simds && seg0 < 32) {
+    nsegs = std::max(1L, 2 * simds / nstrips);
+    seg0 = ((long)d->my + nsegs - 1) / nsegs;
+    waves = nstrips * (((long)d->my + seg0 - 1) / seg0);
+  }
   // A launch just over a whole number m of wavefronts per SIMD (by up to a quarter of the SIMDs)
   // leaves those few SIMDs a round more than the rest, and they set the time; it is cut to m
   // per SIMD instead (FP64 shallow water 1024^2: 1152 wavefronts 81.2 us, 936 wavefronts 66.9 us).
-  const long simds = device_simds();
-  const long seg0 = ((long)d->my + nsegs - 1) / nsegs;
-  const long waves = nstrips * (((long)d->my + seg0 - 1) / seg0);
   if (simds > 0 && waves > simds) {
     const long m = waves / simds;
     if (waves - m * simds <= simds / 4 && m * simds / nstrips >= 1) nsegs = m * simds / nstrips;
