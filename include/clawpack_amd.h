@@ -61,7 +61,9 @@ typedef struct claw2d_desc {
   int32_t bc_lower[2];      /* x, y */
   int32_t bc_upper[2];
   int32_t reflect_index[2]; /* 0-based component negated by a wall in x, y (pyclaw default 1, 2) */
-  int32_t seg_h;            /* rows per marching segment (0 = library default) */
+  int32_t seg_h;            /* rows per marching segment (0 = library default, sized from the
+                               grid, the device's SIMD count and the kernel's occupancy; results
+                               do not depend on it) */
   int32_t dt_variable;      /* solver.dt_variable */
   double dx, dy;
   double grav;              /* problem_data['grav'] (shallow water) */
