@@ -40,10 +40,16 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=500)
     p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--prewarm", type=float, default=0.5,
+                   help="seconds of untimed steps of each kernel before its counted warm-up")
     p.add_argument("--n", type=int, default=4096, help="cells per side")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-head-steps", type=int, default=2,
+                   help="oracle steps at the headline size on one thread (several s each)")
+    p.add_argument("--cpu-head-mt-steps", type=int, default=10,
+                   help="oracle steps at the headline size on all threads")
     p.add_argument("--cpu-n", type=int, default=640)
     p.add_argument("--cpu-steps", type=int, default=100)
     p.add_argument("--cpu-mt-n", type=int, default=1024)
@@ -64,7 +70,7 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--adjoint-n", type=int, default=512)
     p.add_argument("--adjoint-steps", type=int, default=400)
-    p.add_argument("--adjoint-fd", type=int, default=4, help="finite-difference directions checked")
+    p.add_argument("--adjoint-fd", type=int, default=16, help="finite-difference directions checked")
     p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                    help="only run a plain device copy of BYTES bytes 5 times (calibrates the "
                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -108,7 +114,25 @@ def measured_copy_gbs(nbytes=1 << 31, reps=5):
     return 2.0 * n * 8 / (float(np.median(times)) * 1e-3) / 1e9
 
 
-def run_gpu(n, dtype_id, steps, warmup, seed, split=False):
+def prewarm(g, q, dt0, seconds, aux=None):
+    """Runs the grid's own step for at least `seconds` of wall time (clocks and caches settle on
+    the kernel about to be timed), then restores the initial state, so the counted warm-up and the
+    timed steps start from q as if nothing had run.  Returns the seconds spent."""
+    import torch
+    t0 = time.perf_counter()
+    g.init_ctrl(0.0, dt0, target_steps=10 ** 9)
+    while time.perf_counter() - t0 < seconds:
+        g.advance(50)
+        torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    g.upload(q)
+    if aux is not None:
+        g.upload_aux(aux)
+    torch.cuda.synchronize()
+    return el
+
+
+def run_gpu(n, dtype_id, steps, warmup, seed, split=False, prewarm_s=0.0):
     import torch
     from clawpack_amd import problems
     from clawpack_amd.device import DeviceGrid, make_desc
@@ -118,6 +142,7 @@ def run_gpu(n, dtype_id, steps, warmup, seed, split=False):
                   bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
     g = DeviceGrid(d, "cuda:0", split=split)
     g.upload(q)
+    warmed = prewarm(g, q, 1e-4, prewarm_s) if prewarm_s > 0 else 0.0
     del q
     # dt from the initial wave speed, as after pyclaw's first accepted step
     g.init_ctrl(0.0, 1e-4, target_steps=warmup + steps + 10**9)
@@ -139,7 +164,7 @@ def run_gpu(n, dtype_id, steps, warmup, seed, split=False):
     attempts = int(c1.attempts - c0.attempts)
     assert c1.status == 0, "device status %d" % c1.status
     return dict(ms=ms, wall=wall, accepted=accepted, attempts=attempts, cells=n * n,
-                t=c1.t, dt=c1.dt, cfl=c1.cfl)
+                t=c1.t, dt=c1.dt, cfl=c1.cfl, prewarm_s=warmed)
 
 
 def run_batch(ngrids, dtype_id, steps, seq_steps, seed):
@@ -187,7 +212,40 @@ def run_batch(ngrids, dtype_id, steps, seq_steps, seed):
     return out
 
 
-def run_euler(n, steps, warmup, seed):
+def fp32_vs_fp64(make, q, dt, steps, comps, aux=None, split=False):
+    """Error of the FP32 path against the FP64 path after `steps` fixed-dt steps from the same
+    state q (make(dtype_id) -> descriptor): per component, L1 = sum|dq_m| / sum|q_m| and
+    Linf = max|dq_m| / max|q_m|; the worst of each over the components leads the record."""
+    import torch
+    from clawpack_amd.device import DeviceGrid
+    planes = {}
+    for dtype_id in (0, 1):
+        g = DeviceGrid(make(dtype_id), "cuda:0", split=split)
+        g.upload(q)
+        if aux is not None:
+            g.upload_aux(aux)
+        g.init_ctrl(0.0, dt, target_steps=steps)
+        for _ in range(steps):
+            g.enqueue_step(dt)
+        torch.cuda.synchronize()
+        c = g.read_ctrl()
+        assert c.status == 0 and c.cfl <= 1.0, "status %d cfl %g" % (c.status, c.cfl)
+        planes[dtype_id] = g.planes().to(torch.float64).clone()
+        del g
+    a, b = planes[0], planes[1]
+    rec = {}
+    for m, comp in enumerate(comps):
+        diff = (b[m] - a[m]).abs()
+        rec[comp] = {"L1_rel": float(diff.sum() / a[m].abs().sum()),
+                     "Linf_rel": float(diff.max() / a[m].abs().max())}
+    out = {"max_component_Linf_rel": max(r["Linf_rel"] for r in rec.values()),
+           "max_component_L1_rel": max(r["L1_rel"] for r in rec.values()),
+           "per_component": rec, "steps": steps, "dt": dt,
+           "Linf_rel_to_max_q": float((b - a).abs().max() / a.abs().max())}
+    return out
+
+
+def run_euler(n, steps, warmup, seed, prewarm_s=0.0, err_steps=500):
     """Euler gas dynamics: four-quadrant Riemann problem (seeded), euler_4wave with entropy fix,
     unsplit, MC, extrapolation BCs, n x n on [0,1]^2, CFL 0.9 device-controlled dt."""
     import torch
@@ -201,6 +259,8 @@ def run_euler(n, steps, warmup, seed):
                       bc_lower=(1, 1), bc_upper=(1, 1), gamma=1.4)
         g = DeviceGrid(d, "cuda:0")
         g.upload(q)
+        if prewarm_s > 0:
+            prewarm(g, q, 1e-5, prewarm_s)
         g.init_ctrl(0.0, 1e-5, target_steps=10 ** 9)
         g.advance(warmup)
         torch.cuda.synchronize()
@@ -220,12 +280,19 @@ def run_euler(n, steps, warmup, seed):
         out[name] = {"cell_updates_per_s": n * n * acc / (ms * 1e-3), "ms_per_step": ms / acc,
                      "accepted": acc, "attempts": att, "t": c1.t,
                      "roofline_frac": bpc * n * n * att / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if dtype_id == 0:
+            dt_err = 0.8 * c0.dt  # the controller's dt after the warm-up steps, with CFL margin
         del g
+    out["fp32"]["ratio_vs_fp64"] = out["fp32"]["cell_updates_per_s"] / out["fp64"]["cell_updates_per_s"]
+    out["fp32"]["error_vs_fp64"] = fp32_vs_fp64(
+        lambda t: make_desc(1, t, n, n, dx, dy, 4, 4, order=2, transverse_waves=2,
+                            mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_upper=(1, 1), gamma=1.4),
+        q, dt_err, err_steps, ("rho", "rho_u", "rho_v", "E"))
     out["workload"] = "euler_4wave_efix_unsplit_mc_extrap_quadrants_seed%d_%dx%d" % (seed, n, n)
     return out
 
 
-def run_bathy(n, steps, warmup):
+def run_bathy(n, steps, warmup, prewarm_s=0.0, err_steps=500):
     """Shallow water over bathymetry (f-wave solver, aux plane b): the 2D sill problem on
     [-1,1]^2 at n x n, g = 9.81, MC, transverse_waves 2, x extrap / y wall, CFL 0.9 device dt;
     unsplit and split, FP64 and FP32.  Algorithmic bytes per cell-update: q read + q write + the
@@ -244,6 +311,8 @@ def run_bathy(n, steps, warmup):
             g = DeviceGrid(d, "cuda:0", split=split)
             g.upload(q)
             g.upload_aux(aux)
+            if prewarm_s > 0:
+                prewarm(g, q, 1e-5, prewarm_s, aux=aux)
             g.init_ctrl(0.0, 1e-5, target_steps=10 ** 9)
             g.advance(warmup)
             torch.cuda.synchronize()
@@ -263,7 +332,14 @@ def run_bathy(n, steps, warmup):
                 "cell_updates_per_s": n * n * acc / (ms * 1e-3), "ms_per_step": ms / acc,
                 "accepted": acc, "attempts": att, "t": c1.t,
                 "roofline_frac": bpc * n * n * att / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            if dtype_id == 0:
+                dt_err = 0.8 * c0.dt
             del g
+        out[("split_" if split else "unsplit_") + "fp32"]["error_vs_fp64"] = fp32_vs_fp64(
+            lambda t: make_desc(2, t, n, n, dx, dy, 3, 3, order=2, transverse_waves=2,
+                                mthlim=(4, 4, 4), bc_lower=(1, 3), bc_upper=(1, 3), grav=9.81,
+                                dry_tolerance=1e-3),
+            q, dt_err, err_steps, ("h", "hu", "hv"), aux=aux, split=split)
     out["workload"] = "swe_bathymetry_fwave_sill_mc_extrap_wall_%dx%d" % (n, n)
     return out
 
@@ -311,7 +387,10 @@ def run_precision(sizes, steps, seed):
         # the state as a whole: max |dq| / max |q| and sum |dq| / sum |q| over all components
         err["q"] = {"Linf_rel_to_max_q": float(dall.max()) / qmax,
                     "L1_rel": float(dall.sum() / a.abs().sum())}
-        rec["fp32_vs_fp64"] = err
+        rec["fp32_vs_fp64"] = {
+            "max_component_Linf_rel": max(err[c]["Linf_rel"] for c in ("h", "hu", "hv")),
+            "max_component_L1_rel": max(err[c]["L1_rel"] for c in ("h", "hu", "hv")),
+            "per_component": {c: err[c] for c in ("h", "hu", "hv")}, "whole_state": err["q"]}
         rec["steps"] = steps
         rec["dt"] = dt
         out["%dx%d" % (n, n)] = rec
@@ -324,7 +403,9 @@ def run_adjoint(n, nsteps, seed, fd_dirs):
     """Inversion workload: SWE n x n FP64 periodic, nsteps frozen-dt steps from 1 + eta0, h sampled
     at 64 gauges every 10 steps, checkpoints every 20.  Times the forward (steps + checkpoint
     stores + samples) and the backward (segment recompute + reverse steps) and checks the
-    gradient against central differences (eps 1e-6) along fd_dirs seeded unit directions."""
+    gradient against central differences (eps 1e-6) along fd_dirs seeded unit directions, for the
+    MC-limited method and for the unlimited second-order method (no limiter kinks, so the loss is
+    smooth and the finite differences converge to the directional derivative itself)."""
     import numpy as np
     import torch
     from clawpack_amd import problems
@@ -332,57 +413,68 @@ def run_adjoint(n, nsteps, seed, fd_dirs):
     from clawpack_amd.device import make_desc
 
     eta_true, gauges, dx, dy = problems.inversion_workload(n, n, seed=4)
-    d = make_desc(0, 0, n, n, dx, dy, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
-                  bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
-    q_true = torch.tensor(np.stack([1.0 + eta_true, 0 * eta_true, 0 * eta_true]), device="cuda")
-    dts = cfl_dt_sequence(d, q_true, nsteps)
     g = torch.tensor(gauges)
-    with torch.no_grad():
-        obs = evolve_frozen(q_true, d, dts, g, 10, 20)[0][:, 0, :]
     rng = np.random.default_rng(seed)
     eta0 = 0.5 * eta_true + 1e-3 * rng.standard_normal(eta_true.shape)
-
-    def loss_of(eta):
-        z = torch.zeros_like(eta)
-        samples, _ = evolve_frozen(torch.stack([1.0 + eta, z, z]), d, dts, g, 10, 20)
-        return ((samples[:, 0, :] - obs) ** 2).sum()
-
-    fwd, bwd = [], []
-    for rep in range(4):
-        eta = torch.tensor(eta0, device="cuda", requires_grad=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        loss = loss_of(eta)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        loss.backward()
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        if rep:
-            fwd.append(t1 - t0)
-            bwd.append(t2 - t1)
-    grad = eta.grad.cpu().numpy()
-    gnorm = float(np.linalg.norm(grad))
-    rel_dir, rel_norm = [], []
-    eps = 1e-6
-    for k in range(fd_dirs):
+    dirs = []
+    for _ in range(fd_dirs):
         v = rng.standard_normal(eta0.shape)
-        v /= np.linalg.norm(v)
+        dirs.append(v / np.linalg.norm(v))
+    out = {"workload": "swe_inversion_periodic_mc_%dx%d_%dsteps_64gauges_fp64" % (n, n, nsteps),
+           "checkpoint_every": 20}
+    for lim, name in (((4, 4, 4), "mc"), ((0, 0, 0), "unlimited")):
+        d = make_desc(0, 0, n, n, dx, dy, 3, 3, order=2, transverse_waves=2, mthlim=lim,
+                      bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
+        q_true = torch.tensor(np.stack([1.0 + eta_true, 0 * eta_true, 0 * eta_true]), device="cuda")
+        dts = cfl_dt_sequence(d, q_true, nsteps)
         with torch.no_grad():
-            lp = float(loss_of(torch.tensor(eta0 + eps * v, device="cuda")))
-            lm = float(loss_of(torch.tensor(eta0 - eps * v, device="cuda")))
-        fd = (lp - lm) / (2 * eps)
-        ad = float((grad * v).sum())
-        rel_dir.append(abs(fd - ad) / abs(ad))
-        rel_norm.append(abs(fd - ad) / gnorm)
-    f, b = float(np.median(fwd)), float(np.median(bwd))
-    return {"workload": "swe_inversion_periodic_mc_%dx%d_%dsteps_64gauges_fp64" % (n, n, nsteps),
-            "forward_ms": f * 1e3, "backward_ms": b * 1e3, "backward_over_forward": b / f,
-            "total_over_forward": (f + b) / f, "checkpoint_every": 20, "loss": float(loss),
-            "fd": {"eps": eps, "directions": fd_dirs,
-                   "max_abs_err_over_grad_norm": max(rel_norm) if rel_norm else None,
-                   "median_rel_err_vs_directional": float(np.median(rel_dir)) if rel_dir else None,
-                   "max_rel_err_vs_directional": max(rel_dir) if rel_dir else None}}
+            obs = evolve_frozen(q_true, d, dts, g, 10, 20)[0][:, 0, :]
+
+        def loss_of(eta):
+            z = torch.zeros_like(eta)
+            samples, _ = evolve_frozen(torch.stack([1.0 + eta, z, z]), d, dts, g, 10, 20)
+            return ((samples[:, 0, :] - obs) ** 2).sum()
+
+        fwd, bwd = [], []
+        for rep in range(4 if name == "mc" else 1):
+            eta = torch.tensor(eta0, device="cuda", requires_grad=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            loss = loss_of(eta)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if rep:
+                fwd.append(t1 - t0)
+                bwd.append(t2 - t1)
+        grad = eta.grad.cpu().numpy()
+        gnorm = float(np.linalg.norm(grad))
+        rel_dir, rel_norm = [], []
+        eps = 1e-6
+        for v in dirs:
+            with torch.no_grad():
+                lp = float(loss_of(torch.tensor(eta0 + eps * v, device="cuda")))
+                lm = float(loss_of(torch.tensor(eta0 - eps * v, device="cuda")))
+            fd = (lp - lm) / (2 * eps)
+            ad = float((grad * v).sum())
+            rel_dir.append(abs(fd - ad) / abs(ad))
+            rel_norm.append(abs(fd - ad) / gnorm)
+        rec = {"loss": float(loss),
+               "fd": {"eps": eps, "directions": fd_dirs,
+                      "median_rel_err_vs_directional": float(np.median(rel_dir)) if rel_dir else None,
+                      "max_rel_err_vs_directional": max(rel_dir) if rel_dir else None,
+                      "max_abs_err_over_grad_norm": max(rel_norm) if rel_norm else None}}
+        if fwd:
+            f, b = float(np.median(fwd)), float(np.median(bwd))
+            rec.update(forward_ms=f * 1e3, backward_ms=b * 1e3, backward_over_forward=b / f,
+                       total_over_forward=(f + b) / f)
+        if name == "mc":
+            out.update(rec)
+        else:
+            out["unlimited"] = rec
+    return out
 
 
 def cpu_baseline(n, steps, seed, threads=1):
@@ -424,7 +516,7 @@ def main():
     if a.copy_probe:
         copy_probe(a.copy_probe)
         return
-    r64 = run_gpu(a.n, 0, a.steps, a.warmup, a.seed)
+    r64 = run_gpu(a.n, 0, a.steps, a.warmup, a.seed, prewarm_s=a.prewarm)
     progress("fp64 step: %.4f ms/attempt" % (r64["ms"] / r64["attempts"]))
     ms_step = r64["ms"] / r64["accepted"]
     value = r64["cells"] * r64["accepted"] / (r64["ms"] * 1e-3)
@@ -438,6 +530,7 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
+        "prewarm_s": r64["prewarm_s"],
         "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -456,7 +549,7 @@ def main():
     out["roofline"]["measured_copy_gbs"] = copy_gbs
     out["roofline"]["frac_of_measured_copy"] = achieved / copy_gbs
     if not a.no_fp32:
-        r32 = run_gpu(a.n, 1, a.steps, a.warmup, a.seed)
+        r32 = run_gpu(a.n, 1, a.steps, a.warmup, a.seed, prewarm_s=a.prewarm)
         v32 = r32["cells"] * r32["accepted"] / (r32["ms"] * 1e-3)
         progress("fp32 step: %.4f ms/attempt" % (r32["ms"] / r32["attempts"]))
         out["fp32"] = {"value": v32, "ms_per_step": r32["ms"] / r32["accepted"],
@@ -465,7 +558,7 @@ def main():
     if not a.no_split:
         sp = {}
         for dt_id, name, bpc in ((0, "fp64", 48.0), (1, "fp32", 24.0)):
-            r = run_gpu(a.n, dt_id, a.steps, a.warmup, a.seed, split=True)
+            r = run_gpu(a.n, dt_id, a.steps, a.warmup, a.seed, split=True, prewarm_s=a.prewarm)
             sp[name] = {"cell_updates_per_s": r["cells"] * r["accepted"] / (r["ms"] * 1e-3),
                         "ms_per_step": r["ms"] / r["accepted"],
                         "roofline_frac": bpc * r["cells"] * r["attempts"] / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
@@ -477,10 +570,10 @@ def main():
                         "fp32": run_batch(a.batch_grids, 1, a.batch_steps, a.seq_steps, a.seed)}
         progress("batch done")
     if not a.no_euler:
-        out["euler"] = run_euler(a.euler_n, a.euler_steps, 10, 1)
+        out["euler"] = run_euler(a.euler_n, a.euler_steps, 10, 1, prewarm_s=a.prewarm)
         progress("euler done")
     if not a.no_bathy:
-        out["bathymetry"] = run_bathy(a.n, a.bathy_steps, a.warmup)
+        out["bathymetry"] = run_bathy(a.n, a.bathy_steps, a.warmup, prewarm_s=a.prewarm)
         progress("bathymetry done")
     if not a.no_precision:
         out["precision"] = run_precision([int(x) for x in a.precision_sizes.split(",")],
@@ -490,18 +583,27 @@ def main():
         out["adjoint"] = run_adjoint(a.adjoint_n, a.adjoint_steps, a.seed, a.adjoint_fd)
         progress("adjoint done")
     if not a.no_cpu:
-        rate1, el1 = cpu_baseline(a.cpu_n, a.cpu_steps, a.seed, 1)
-        progress("cpu 1 thread done")
         nth = host_threads()
-        rate_mt, el_mt = cpu_baseline(a.cpu_mt_n, a.cpu_mt_steps, a.seed, nth)
-        progress("cpu %d threads done" % nth)
+        rate_mt, el_mt = cpu_baseline(a.n, a.cpu_head_mt_steps, a.seed, nth)
+        progress("cpu %d threads at the headline size done" % nth)
+        rate1, el1 = cpu_baseline(a.n, a.cpu_head_steps, a.seed, 1)
+        progress("cpu 1 thread at the headline size done")
+        rate1s, el1s = cpu_baseline(a.cpu_n, a.cpu_steps, a.seed, 1)
+        rate_mts, el_mts = cpu_baseline(a.cpu_mt_n, a.cpu_mt_steps, a.seed, nth)
+        progress("cpu smaller samples done")
         out["cpu_baseline"] = {
             "value": rate_mt, "unit": "cell-updates/s", "cores": nth, "kind": "port",
             "sample": "C oracle (CPU restatement of step2/flux2/rpn2/rpt2, not the reference binary), "
-                      "same workload at %dx%d for %d steps on %d OpenMP threads (%.1f s wall)"
-                      % (a.cpu_mt_n, a.cpu_mt_n, a.cpu_mt_steps, nth, el_mt),
+                      "the headline workload (%dx%d, FP64) for %d steps on %d OpenMP threads "
+                      "(%.1f s wall)" % (a.n, a.n, a.cpu_head_mt_steps, nth, el_mt),
             "single_core": {"value": rate1, "cores": 1,
-                            "sample": "%dx%d for %d steps (%.1f s)" % (a.cpu_n, a.cpu_n, a.cpu_steps, el1)}}
+                            "sample": "the headline workload (%dx%d) for %d steps (%.1f s)"
+                                      % (a.n, a.n, a.cpu_head_steps, el1)},
+            "smaller_samples": {
+                "single_core": {"value": rate1s, "sample": "%dx%d for %d steps (%.1f s)"
+                                % (a.cpu_n, a.cpu_n, a.cpu_steps, el1s)},
+                "threads": {"value": rate_mts, "cores": nth, "sample": "%dx%d for %d steps (%.1f s)"
+                            % (a.cpu_mt_n, a.cpu_mt_n, a.cpu_mt_steps, el_mts)}}}
     print(json.dumps(out), flush=True)
 
 

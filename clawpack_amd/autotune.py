@@ -50,20 +50,27 @@ def _size_class(n):
     return 1 << max(4, int(math.ceil(math.log2(max(int(n), 1)))))
 
 
-def key(desc, kind="grid"):
+def key(desc, kind="grid", split=False):
+    """Cache key: everything that selects a different kernel or launch shape (device, runtime,
+    solver, precision, size class, split or unsplit step, order, transverse mode, the limiter of
+    every wave, aux planes)."""
     dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "none"
     hip = getattr(torch.version, "hip", None) or "unknown"
     return "|".join([kind, dev, "hip" + str(hip), "solver%d" % desc.solver,
                      "fp64" if desc.dtype == _lib.FP64 else "fp32",
                      "%dx%d" % (_size_class(desc.mx), _size_class(desc.my)),
-                     "lim%d" % desc.mthlim[0]])
+                     "split" if split else "unsplit",
+                     "order%d" % desc.order, "tw%d" % desc.transverse_waves,
+                     "lim" + "".join(str(int(desc.mthlim[k])) for k in range(desc.num_waves)),
+                     "aux%d" % desc.num_aux])
 
 
-def time_seg_h(desc, q, seg_h, dt, steps=10, warmup=2):
-    """ms per fixed-dt step of the single-grid kernel with desc.seg_h = seg_h."""
+def time_seg_h(desc, q, seg_h, dt, steps=10, warmup=2, split=False):
+    """ms per fixed-dt step of the single-grid kernel (split: the step2ds kernel) with
+    desc.seg_h = seg_h."""
     d = type(desc).from_buffer_copy(bytes(desc))
     d.seg_h = int(seg_h)
-    g = DeviceGrid(d)
+    g = DeviceGrid(d, split=split)
     g.upload(q)
     g.init_ctrl(0.0, dt)
     for _ in range(warmup):
@@ -78,10 +85,10 @@ def time_seg_h(desc, q, seg_h, dt, steps=10, warmup=2):
     return e0.elapsed_time(e1) / steps
 
 
-def tune_grid(desc, q, dt, candidates=DEFAULT_CANDIDATES, steps=10, use_cache=True):
-    """Best seg_h for this descriptor (0 = the occupancy-derived default).  Returns (seg_h, table
-    of ms per step by candidate)."""
-    k = key(desc)
+def tune_grid(desc, q, dt, candidates=DEFAULT_CANDIDATES, steps=10, use_cache=True, split=False):
+    """Best seg_h for this descriptor (0 = the occupancy-derived default) and step form.  Returns
+    (seg_h, table of ms per step by candidate)."""
+    k = key(desc, split=split)
     table = _load() if use_cache else {}
     if use_cache and k in table:
         return int(table[k]["seg_h"]), table[k]["ms"]
@@ -89,7 +96,7 @@ def tune_grid(desc, q, dt, candidates=DEFAULT_CANDIDATES, steps=10, use_cache=Tr
     for c in candidates:
         if c and c > desc.my:
             continue
-        ms[str(c)] = time_seg_h(desc, q, c, dt, steps=steps)
+        ms[str(c)] = time_seg_h(desc, q, c, dt, steps=steps, split=split)
     best = int(min(ms, key=ms.get))
     if use_cache:
         table = _load()
@@ -127,9 +134,10 @@ def tune_batch(desc, grids, candidates=(16, 32, 48, 64, 128), attempts=5, use_ca
     return best, ms
 
 
-def tuned_seg_h(desc, q=None, dt=None):
-    """Cached seg_h for desc (tuning on a synthetic state of the right shape if needed)."""
-    k = key(desc)
+def tuned_seg_h(desc, q=None, dt=None, split=False):
+    """Cached seg_h for desc and step form (tuning on a synthetic state of the right shape if
+    needed)."""
+    k = key(desc, split=split)
     table = _load()
     if k in table:
         return int(table[k]["seg_h"])
@@ -140,4 +148,4 @@ def tuned_seg_h(desc, q=None, dt=None):
             q[3] = 2.5
     if dt is None:
         dt = 0.1 * min(desc.dx, desc.dy)
-    return tune_grid(desc, q, dt)[0]
+    return tune_grid(desc, q, dt, split=split)[0]

@@ -645,7 +645,8 @@ __device__ __forceinline__ void control_one(claw2d_ctrl* c, double dx, double dy
     return;
   }
   if (c->attempts >= c->max_steps) {
-    if (c->numsteps == c->max_steps) c->status |= CLAW_STATUS_MAX_STEPS;
+    // pyclaw's evolve_to_time raises only with a variable dt and every attempt accepted
+    if (dtvar && c->numsteps == c->max_steps) c->status |= CLAW_STATUS_MAX_STEPS;
     c->done = 1;
     return;
   }

@@ -115,22 +115,19 @@ class ClawSolver2D(Solver):
                 "kernel_language=%r is not available in clawpack_amd; the only compute path is "
                 "'HIP' (the reference's f2py Fortran/Python kernels are not part of this package)"
                 % (self.kernel_language,))
-        if self.rp is None or not hasattr(self.rp, "kernel_id"):
-            raise NotImplementedError("Riemann solver not available on the HIP path: use "
-                                      "riemann.shallow_roe_with_efix_2D, riemann.euler_4wave_2D or "
-                                      "riemann.shallow_bathymetry_fwave_2D")
-        rp_fwave = bool(getattr(self.rp, "fwave", False))
+        pol = self._policy()
+        rp_fwave = pol.fwave
         if bool(self.fwave) != rp_fwave:
             raise ValueError("solver.fwave must be %s for %s (its Riemann solver returns %s)"
-                             % (rp_fwave, self.rp.__name__, "f-waves" if rp_fwave else "waves"))
-        rp_aux = int(getattr(self.rp, "num_aux", 0))
+                             % (rp_fwave, pol.name, "f-waves" if rp_fwave else "waves"))
+        rp_aux = pol.num_aux
         if state.index_capa >= 0:
             raise NotImplementedError("a capacity function (index_capa) is not supported on the HIP path")
         if rp_aux == 0 and state.num_aux > 0:
             raise NotImplementedError("%s reads no aux arrays; aux is only supported with "
-                                      "shallow_bathymetry_fwave_2D on the HIP path" % self.rp.__name__)
+                                      "shallow_bathymetry_fwave_2D on the HIP path" % pol.name)
         if state.num_aux < rp_aux:
-            raise ValueError("%s needs num_aux >= %d (aux[0] = bathymetry)" % (self.rp.__name__, rp_aux))
+            raise ValueError("%s needs num_aux >= %d (aux[0] = bathymetry)" % (pol.name, rp_aux))
         if state.num_aux > 0:
             for k in range(2):
                 lo, hi = self.aux_bc_lower[k], self.aux_bc_upper[k]
@@ -162,19 +159,33 @@ class ClawSolver2D(Solver):
                 raise Exception("Upper custom BC function has not been set.")
         if self.precision not in ("fp64", "fp32"):
             raise ValueError("solver.precision must be 'fp64' or 'fp32'")
-        if state.num_eqn != self.rp.num_eqn:
+        if state.num_eqn != pol.num_eqn:
             raise ValueError("state.num_eqn does not match the Riemann solver")
-        for key in self.rp.problem_data_keys:
+        for key in pol.problem_data_keys:
             if key not in state.problem_data:
-                raise KeyError("state.problem_data[%r] must be set for %s" % (key, self.rp.__name__))
+                raise KeyError("state.problem_data[%r] must be set for %s" % (key, pol.name))
+
+    def _policy(self):
+        """The HIP solver for solver.rp, chosen by the module's name as pyclaw's riemann.static
+        lookup does: this package's riemann modules and upstream's clawpack.riemann modules of the
+        same name select the same kernel."""
+        from ..riemann import static
+        pol = None if self.rp is None else static.policy(self.rp)
+        if pol is None:
+            raise NotImplementedError(
+                "Riemann solver %r not available on the HIP path: use shallow_roe_with_efix_2D, "
+                "euler_4wave_2D or shallow_bathymetry_fwave_2D (from clawpack_amd.riemann or "
+                "clawpack.riemann)" % (static.solver_name(self.rp) if self.rp is not None else None,))
+        return pol
 
     def _make_desc(self, state):
         mx, my = state.grid.num_cells
         dx, dy = state.grid.delta
         pd = state.problem_data
+        pol = self._policy()
         return make_desc(
-            self.rp.kernel_id, _lib.FP64 if self.precision == "fp64" else _lib.FP32, mx, my, dx, dy,
-            self.rp.num_eqn, self.rp.num_waves, order=self.order,
+            pol.kernel_id, _lib.FP64 if self.precision == "fp64" else _lib.FP32, mx, my, dx, dy,
+            pol.num_eqn, pol.num_waves, order=self.order,
             transverse_waves=self.transverse_waves, mthlim=self._mthlim(),
             # a custom side's ghost cells come from the user's function through the padded step,
             # which does not read the state's BC codes; the descriptor carries extrap there
@@ -201,7 +212,7 @@ class ClawSolver2D(Solver):
         desc = self._make_desc(state)
         if self.autotune and self.seg_h == 0:
             from .. import autotune
-            desc.seg_h = autotune.tuned_seg_h(desc)
+            desc.seg_h = autotune.tuned_seg_h(desc, split=bool(self.dimensional_split))
         key = (desc.mx, desc.my, desc.num_eqn, desc.dtype, desc.num_aux)
         if self._grid is None or self._desc_key != key:
             self._grid = DeviceGrid(desc)
@@ -337,7 +348,7 @@ class ClawSolver2D(Solver):
         if c.status & _lib.STATUS_CFL_FIXED_DT:
             raise Exception("CFL too large, giving up!")
         self.dt = c.dt
-        if (c.status & _lib.STATUS_MAX_STEPS) or (not take_one_step and not c.done):
+        if c.status & _lib.STATUS_MAX_STEPS:
             raise Exception("Maximum number of timesteps have been taken")
         return self.status
 
@@ -395,10 +406,12 @@ class ClawSolver2D(Solver):
             if not (cfl > self.cfl_max):
                 cflmax = max(cfl, cflmax)
                 numsteps += 1
-                t = t + dt if self.dt_variable else tstart + numsteps * dt
                 g.cur = 1 - g.cur
                 if self.step_source is not None:
+                    # pyclaw's Solver.step applies the source before evolve_to_time advances
+                    # solution.t, so the source of an accepted step sees the step's start time
                     self.step_source(self, ds.bind(g.q[g.cur], t), dt / 2.0 if strang else dt)
+                t = t + dt if self.dt_variable else tstart + numsteps * dt
             else:
                 rejected += 1
                 if not self.dt_variable:
@@ -428,7 +441,9 @@ class ClawSolver2D(Solver):
         self.status["numsteps"] = numsteps
         self.status["rejected"] = rejected
         self.dt = dt  # the dt of the last attempt (pyclaw leaves the loop before choosing a new one)
-        if not done:
+        # pyclaw raises only when every one of the max_steps attempts was accepted and tend is not
+        # reached; attempts rejected on the way end the loop silently short of tend
+        if not done and self.dt_variable and t < tend and numsteps == self.max_steps:
             raise Exception("Maximum number of timesteps have been taken")
         return self.status
 

@@ -974,7 +974,10 @@ int orc_evolve_aux(const orc_desc *d, const orc_ctrl *c, double *q, const double
             }
         }
     }
-    if (nsteps_target > 0 ? num_steps < nsteps_target : (c->dt_variable && st->t < tend))
+    /* Solver.evolve_to_time raises "Maximum number of timesteps" only with a variable dt, short of
+     * tend, after max_steps accepted steps: rejected attempts end the loop silently */
+    if (nsteps_target > 0 ? num_steps < nsteps_target
+                          : (c->dt_variable && st->t < tend && num_steps == c->max_steps))
         st->error = 1;
     free(qbackup);
     return 0;

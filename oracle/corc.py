@@ -325,7 +325,8 @@ def apply_bcs(d, c, q, t=0.0, user_bc_lower=None, user_bc_upper=None, auxbc=None
 
 
 def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, nsteps=None,
-                       tend=None, t0=0.0, dt=None, aux=None, user_bc_lower=None, user_bc_upper=None):
+                       tend=None, t0=0.0, dt=None, aux=None, user_bc_lower=None, user_bc_upper=None,
+                       source_t=False):
     """Solver.evolve_to_time with Solver.step's source splitting (upstream pyclaw/src/pyclaw/
     solver.py, absent from the reference tree): per attempt before_step(q, t), for Strang
     splitting (source_split 2) a half step source(q, dt/2) before the hyperbolic step, and after
@@ -335,7 +336,9 @@ def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, n
     functions and the step is orc_step2 on that qbc, or with dimensional splitting the x-sweeps of
     orc_step2ds, apply_bcs on q* and the y-sweeps.  The accept/reject, t and dt rules are orc_evolve's; nsteps stops after that many
     accepted steps (no clipping to tend), tend runs to that time.  The hyperbolic step is
-    orc_step_hyperbolic."""
+    orc_step_hyperbolic.  With source_t the source is called as source(q, dt, t), t being the
+    attempt's start time in both of its calls: Solver.step runs before evolve_to_time advances
+    solution.t."""
     q = np.array(q, dtype=np.float64, order="F", copy=True)
     t = tstart = t0
     dt = c.dt_initial if dt is None else dt
@@ -350,8 +353,11 @@ def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, n
         qbackup = q.copy(order="F")
         if before_step is not None:
             before_step(q, t)
+        src = source
+        if source is not None and source_t:
+            src = lambda qq, h, t_=t: source(qq, h, t_)  # noqa: E731
         if source is not None and source_split == 2:
-            source(q, dt / 2.0)
+            src(q, dt / 2.0)
         if BC_CUSTOM in tuple(c.bc_lower) + tuple(c.bc_upper):
             g = d.num_ghost
             auxbc = None if aux is None else fill_aux_ghosts(d, c, aux)
@@ -372,9 +378,9 @@ def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, n
             q = qn
             st["cflmax"] = max(cfl, st["cflmax"])
             st["numsteps"] += 1
-            t = t + dt if c.dt_variable else tstart + st["numsteps"] * dt
             if source is not None:
-                source(q, dt / 2.0 if source_split == 2 else dt)
+                src(q, dt / 2.0 if source_split == 2 else dt)
+            t = t + dt if c.dt_variable else tstart + st["numsteps"] * dt
         else:
             st["rejected"] += 1
             q = qbackup
@@ -391,6 +397,8 @@ def evolve_with_source(d, c, q, source=None, source_split=1, before_step=None, n
             else:
                 dt = c.dt_max
     else:
-        st["error"] = 1
+        # as orc_evolve: short of tend only every attempt accepted is an error (Solver.evolve_to_time)
+        if not to_tend or (c.dt_variable and t < tend and st["numsteps"] == c.max_steps):
+            st["error"] = 1
     st["t"], st["dt"] = t, dt
     return q, st

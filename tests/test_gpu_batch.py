@@ -85,3 +85,62 @@ def test_graph_replays_match_eager():
             assert ce[g].attempts == cg[g].attempts == per_graph * (r + 1)
             assert ce[g].t == cg[g].t
             assert np.array_equal(eager.download(g, ce), graphed.download(g, cg)), (r, g)
+
+
+def _batch_method(solver, dtype, bc_lower, bc_upper, mthlim, ngrids=10, seed=11):
+    """Grids of uneven sizes for one method: Euler quadrants (seed 1, screened for positivity) or
+    16-hump shallow water, with the given BCs and per-wave limiters."""
+    from clawpack_amd.batch import BatchedGrids
+    from clawpack_amd.device import make_desc
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(24, 150, size=(ngrids, 2))
+    if solver == "euler":
+        grids = [problems.euler_quadrants(int(mx), int(my), 1) for mx, my in sizes]
+        sid, neq = 1, 4
+    else:
+        grids = [problems.gaussian_humps(int(mx), int(my), seed + g)
+                 for g, (mx, my) in enumerate(sizes)]
+        sid, neq = 0, 3
+    d = make_desc(sid, dtype, 8, 8, 1.0, 1.0, neq, neq, bc_lower=bc_lower, bc_upper=bc_upper,
+                  grav=1.0, gamma=1.4, mthlim=mthlim)
+    return grids, (lambda: BatchedGrids(d, grids))
+
+
+# (solver, bc_lower, bc_upper, mthlim): Euler with the specialised MC kernel under extrap/wall
+# sides, and the run-time-limiter kernel (mixed per-wave limiters) for both solvers with walls
+BATCH_METHODS = [
+    ("euler", (1, 3), (3, 1), (4, 4, 4, 4)),
+    ("euler", (3, 3), (3, 3), (1, 4, 2, 3)),
+    ("swe", (3, 1), (1, 3), (2, 4, 1)),
+]
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("solver,bcl,bcu,lim", BATCH_METHODS)
+def test_batched_methods_equal_sequential_and_oracle(solver, bcl, bcu, lim, dtype):
+    """Batched Euler and run-time-limiter kernels with non-periodic BCs: bit for bit equal to one
+    launch per grid, and per grid equal to the oracle (FP64 to 1e-12 max|q|; FP32 to 1e-4, the
+    single-precision rounding over 10 steps of shocks)."""
+    print("case batch", solver, bcl, bcu, lim, "dtype", dtype, flush=True)
+    grids, make = _batch_method(solver, dtype, bcl, bcu, lim)
+    b1, b2 = make(), make()
+    steps = 10
+    for b in (b1, b2):
+        b.init(0.0, 0.1, target_steps=steps)
+    b1.advance(steps + 3)
+    b2.advance_sequential(steps + 3)
+    torch.cuda.synchronize()
+    print("  advances done", flush=True)
+    c1, c2 = b1.read_ctrls(), b2.read_ctrls()
+    for g, (q, dx, dy) in enumerate(grids):
+        assert c1[g].numsteps == c2[g].numsteps == steps
+        assert c1[g].t == c2[g].t and c1[g].dt == c2[g].dt
+        qb = b1.download(g, c1)
+        assert np.array_equal(qb, b2.download(g, c2)), g
+        mx, my = q.shape[1:]
+        d = corc.make_desc(solver, mx, my, dx, dy, grav=1.0, gamma=1.4, mthlim=lim)
+        c = corc.make_ctrl(bc_lower=bcl, bc_upper=bcu)
+        qo, st = corc.evolve(d, c, q, nsteps=steps)
+        assert c1[g].rejected == st["rejected"]
+        err = np.abs(qb - qo).max() / np.abs(q).max()
+        assert err <= (1e-12 if dtype == 0 else 1e-4), (g, err)

@@ -73,3 +73,53 @@ def test_fp32_step_matches_oracle_loosely():
     err = np.abs(g.download() - qo).max() / np.abs(q).max()
     assert err <= 2e-6, err
     assert abs(cfl - cflo) <= 1e-6 * cflo
+
+
+@pytest.mark.parametrize("bc", [(1, 1), (2, 2), (1, 3)])
+def test_euler_fp32_specialised_kernel_matches_oracle(bc):
+    """The FP32 Euler kernel that the bench times (uniform MC, order 2, transverse_waves 2: the
+    method fixed at compile time) against the FP64 oracle: one step of random smooth data, and 20
+    controller steps of the four-quadrant problem at 192^2 with the oracle's dt sequence.  FP32
+    rounding bounds: 2e-6 max|q| for the step, 1e-4 after 20 steps through the shocks."""
+    from clawpack_amd.device import DeviceGrid, make_desc
+    print("case euler fp32 specialised", bc, flush=True)
+    q = problems.random_euler(70, 33, 4)
+    d = make_desc(1, 1, 70, 33, 0.1, 0.1, 4, 4, bc_lower=bc, bc_upper=bc, gamma=1.4,
+                  mthlim=(4, 4, 4, 4), order=2, transverse_waves=2)
+    g = DeviceGrid(d)
+    g.upload(q)
+    g.init_ctrl(0.0, 0.005)
+    cfl, _ = g.step(0.005)
+    od = corc.make_desc("euler", 70, 33, 0.1, 0.1, gamma=1.4)
+    qo, cflo = corc.step_hyperbolic(od, corc.make_ctrl(bc_lower=bc, bc_upper=bc), q, 0.005)
+    err = np.abs(g.download() - qo).max() / np.abs(q).max()
+    print("  one step: rel err %.3e" % err, flush=True)
+    assert err <= 2e-6, err
+    assert abs(cfl - cflo) <= 1e-6 * cflo
+    n, steps = 192, 20
+    q, dx, dy = problems.euler_quadrants(n, n, 1)
+    od = corc.make_desc("euler", n, n, dx, dy, gamma=1.4)
+    c = corc.make_ctrl(bc_lower=bc, bc_upper=bc)
+    qo, st = corc.evolve(od, c, q, nsteps=steps)
+    d = make_desc(1, 1, n, n, dx, dy, 4, 4, bc_lower=bc, bc_upper=bc, gamma=1.4)
+    g = DeviceGrid(d)
+    g.upload(q)
+    dts = np.diff(np.r_[0.0, _oracle_times(od, c, q, steps)])
+    g.init_ctrl(0.0, float(dts[0]), target_steps=steps)
+    for dt in dts:
+        g.enqueue_step(float(dt))
+    err = np.abs(g.download() - qo).max() / np.abs(q).max()
+    print("  %d steps: rel err %.3e" % (steps, err), flush=True)
+    assert err <= 1e-4, err
+
+
+def _oracle_times(od, c, q, steps):
+    """The oracle's accepted step times t_1..t_steps (one controller step per call)."""
+    out, qq, t, dt = [], q, 0.0, c.dt_initial
+    while len(out) < steps:
+        qn, cfl = corc.step_hyperbolic(od, c, qq, dt)
+        if cfl <= c.cfl_max:
+            qq, t = qn, t + dt
+            out.append(t)
+        dt = min(c.dt_max, dt * c.cfl_desired / cfl)
+    return np.array(out)

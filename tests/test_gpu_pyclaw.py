@@ -190,3 +190,34 @@ def test_controller_writes_ascii_frames(tmp_path):
         back = pyclaw.Solution(k, path=str(tmp_path))
         assert abs(back.t - claw.frames[k].t) <= 1e-8 * max(1.0, claw.frames[k].t)
         np.testing.assert_allclose(back.state.q, claw.frames[k].state.q, rtol=1e-8, atol=1e-12)
+
+
+def test_upstream_riemann_module_object_runs_the_same_step():
+    """A module object shaped like upstream's f2py-built clawpack.riemann.euler_4wave_2D (its name
+    and rpn2/rpt2 procedure objects, nothing else) drives the solver to the same state, bit for bit,
+    as this package's module: the solver is chosen by the module's name."""
+    import types
+    print("case upstream module object", flush=True)
+    up = types.ModuleType("clawpack.riemann.euler_4wave_2D")
+    up.rpn2 = up.rpt2 = object()
+    n = 96
+    q0, dx, dy = problems.euler_quadrants(n, n, 1)
+    out = []
+    for rp in (riemann.euler_4wave_2D, up):
+        solver = pyclaw.ClawSolver2D(rp)
+        solver.dimensional_split = False
+        solver.transverse_waves = 2
+        solver.limiters = pyclaw.limiters.tvd.MC
+        solver.bc_lower = solver.bc_upper = [pyclaw.BC.extrap] * 2
+        domain = pyclaw.Domain((0.0, 0.0), (1.0, 1.0), (n, n))
+        state = pyclaw.State(domain, solver.num_eqn)
+        state.problem_data["gamma"] = 1.4
+        state.q[...] = q0
+        sol = pyclaw.Solution(state, domain)
+        solver.evolve_to_time(sol, 0.05)
+        out.append((state.q.copy(), sol.t, solver.status["numsteps"]))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1:] == out[1][1:]
+    od = corc.make_desc("euler", n, n, dx, dy, gamma=1.4)
+    qo, st = corc.evolve(od, corc.make_ctrl(), q0, tend=0.05)
+    assert st["numsteps"] == out[1][2]
+    assert np.abs(out[1][0] - qo).max() <= 1e-12 * np.abs(q0).max()

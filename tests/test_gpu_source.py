@@ -118,3 +118,61 @@ def test_bathymetry_friction_source_reads_aux():
     qo, st = corc.evolve_with_source(d, c, q0, source=friction_np, tend=0.3, aux=aux)
     assert solver.status["numsteps"] == st["numsteps"]
     assert np.abs(state.q - qo).max() <= 1e-12 * np.abs(q0).max()
+
+
+@pytest.mark.parametrize("source_split", [1, 2])
+def test_time_dependent_source_sees_the_step_start_time(source_split):
+    """A source that reads state.t gets the attempt's start time t_n (pyclaw advances solution.t
+    after Solver.step): on a uniform state at rest dh/dt = t adds dt * t_n per step, the oracle's
+    run with the same source agrees, and the closed form h_0 + sum dt t_n holds."""
+    n = 48
+    print("case time source", source_split, flush=True)
+    solver, sol, q0, dx, dy = dam_break_solution(n, False)
+    sol.state.q[0] = 1.0
+    q0 = sol.state.q.copy()
+    solver.step_source = lambda s, state, dt: state.q[0].add_(dt * state.t)
+    solver.source_split = source_split
+    solver.evolve_to_time(sol, 0.5)
+    d = corc.make_desc("swe", n, n, dx, dy, grav=1.0)
+    c = corc.make_ctrl(bc_lower=(2, 2), bc_upper=(2, 2))
+
+    def src(q, dt, t):
+        q[0] += dt * t
+
+    qo, st = corc.evolve_with_source(d, c, q0, source=src, source_split=source_split, tend=0.5,
+                                     source_t=True)
+    assert solver.status["numsteps"] == st["numsteps"] and abs(sol.t - st["t"]) <= 1e-14
+    assert np.abs(sol.state.q - qo).max() <= 1e-13
+    # every step is accepted at a CFL-chosen dt; without momentum the left sum is the exact
+    # integral's left Riemann sum, below t^2/2 by sum(dt^2)/2
+    assert 1.0 < sol.state.q[0, 0, 0] < 1.0 + 0.5 * 0.5 ** 2
+
+
+@pytest.mark.parametrize("callbacks", [False, True])
+def test_max_steps_spent_on_rejected_attempts(callbacks):
+    """pyclaw raises "Maximum number of timesteps" only when all max_steps attempts were accepted
+    short of tend.  The 200^2 dam break rejects its dt_initial = 0.1 attempt: with max_steps = 3
+    it takes 2 steps and returns short of tend, on the device loop and on the host loop that a
+    before_step callback selects, like the oracle; a dt that is never rejected raises."""
+    print("case max_steps callbacks", callbacks, flush=True)
+    q0, dx, dy = problems.radial_dam_break(200, 200)
+    d = corc.make_desc("swe", 200, 200, dx, dy, grav=1.0)
+    _, st = corc.evolve(d, corc.make_ctrl(max_steps=3), q0, tend=1.0)
+    assert st["error"] == 0 and st["numsteps"] == 2 and st["rejected"] == 1
+    for dt0, raises in ((0.1, False), (0.005, True)):
+        solver, sol, _, _, _ = dam_break_solution(200, False)
+        solver.bc_lower = [pyclaw.BC.extrap] * 2
+        solver.bc_upper = [pyclaw.BC.extrap] * 2
+        sol.state.q[...] = q0
+        solver.max_steps = 3
+        solver.dt_initial = dt0
+        if callbacks:
+            solver.before_step = lambda s, state: None
+        if raises:
+            with pytest.raises(Exception, match="Maximum number of timesteps"):
+                solver.evolve_to_time(sol, 1.0)
+            assert solver.status["numsteps"] == 3
+        else:
+            solver.evolve_to_time(sol, 1.0)
+            assert solver.status["numsteps"] == 2 and solver.status["rejected"] == 1
+            assert abs(sol.t - st["t"]) <= 1e-14 and sol.t < 1.0

@@ -146,3 +146,44 @@ def test_aux_with_a_solver_that_reads_none_is_refused():
     state.aux = np.zeros((1, 20, 10), order="F")
     with pytest.raises(NotImplementedError, match="aux"):
         solver.setup(sol)
+
+
+def upstream_module(name):
+    """A stand-in for an f2py-built upstream Riemann module (clawpack.riemann.<name>): only what
+    such a module carries, the module name and the rpn2/rpt2 procedure objects with their _cpointer
+    (here placeholders; the HIP path never calls them)."""
+    import types
+    m = types.ModuleType("clawpack.riemann." + name)
+
+    class _Proc:
+        _cpointer = None
+
+    m.rpn2, m.rpt2 = _Proc(), _Proc()
+    return m
+
+
+@pytest.mark.parametrize("name", ["shallow_roe_with_efix_2D", "euler_4wave_2D",
+                                  "shallow_bathymetry_fwave_2D"])
+def test_upstream_riemann_module_selects_the_same_solver(name):
+    """An upstream module object selects the HIP solver by its name, as riemann.static does: the
+    same num_eqn/num_waves and the same kernel descriptor, byte for byte, as this package's module."""
+    ours = getattr(riemann, name)
+    up = upstream_module(name)
+    a, b = pyclaw.ClawSolver2D(ours), pyclaw.ClawSolver2D(up)
+    assert (a.num_eqn, a.num_waves) == (b.num_eqn, b.num_waves) == (ours.num_eqn, ours.num_waves)
+    assert a._policy() == b._policy() and a._policy().kernel_id == ours.kernel_id
+    domain = pyclaw.Domain((0.0, 0.0), (1.0, 1.0), (12, 10))
+    state = pyclaw.State(domain, ours.num_eqn, getattr(ours, "num_aux", 0))
+    state.problem_data.update(grav=1.0, gamma=1.4, dry_tolerance=1e-3)
+    for s in (a, b):
+        s.fwave = bool(getattr(ours, "fwave", False))
+        s.bc_lower = s.bc_upper = s.aux_bc_lower = s.aux_bc_upper = [pyclaw.BC.extrap] * 2
+        s._check_supported(state)
+    assert bytes(a._make_desc(state)) == bytes(b._make_desc(state))
+
+
+def test_unknown_riemann_module_is_refused():
+    solver, state, domain, sol = make()
+    solver.rp = upstream_module("acoustics_2D")
+    with pytest.raises(NotImplementedError, match="acoustics_2D"):
+        solver.setup(sol)

@@ -165,3 +165,40 @@ def test_custom_inflow_boundary_sends_a_bore(split):
     mass0, mass1 = q[0].sum() / (mx * my), qn[0].sum() / (mx * my)
     assert 0.5 * 0.6 * 0.3 < mass1 - mass0 < 1.5 * 0.6 * 0.3
     assert np.array_equal(qn[:, -10:], q[:, -10:])
+
+
+@pytest.mark.parametrize("split_src", [1, 2])
+def test_time_dependent_source_sees_the_step_start_time(split_src):
+    """Solver.step runs before evolve_to_time advances solution.t, so a source that reads the time
+    sees t_n in both of its calls.  On a uniform periodic state at rest (a fixed point of the
+    hyperbolic step) the source dh/dt = t then adds dt * t_n per step, the left Riemann sum:
+    h_N = h_0 + dt^2 N (N - 1) / 2 for a fixed dt; seeing t_{n+1} would add dt^2 N (N + 1) / 2."""
+    mx, my, dt, n = 12, 10, 0.01, 30
+    q = np.zeros((3, mx, my), order="F")
+    q[0] = 1.0
+    d = corc.make_desc("swe", mx, my, 1.0 / mx, 1.0 / my, grav=1.0)
+    c = corc.make_ctrl(bc_lower=(2, 2), bc_upper=(2, 2), dt_initial=dt, dt_variable=False)
+
+    def src(qq, h, t):
+        qq[0] += h * t
+
+    qn, st = corc.evolve_with_source(d, c, q, source=src, source_split=split_src, nsteps=n,
+                                     source_t=True)
+    assert st["numsteps"] == n and abs(st["t"] - n * dt) < 1e-15
+    assert np.abs(qn[0] - (1.0 + dt * dt * n * (n - 1) / 2)).max() <= 1e-14
+    assert np.all(qn[1:] == 0.0)
+
+
+def test_max_steps_spent_on_rejected_attempts_is_not_an_error():
+    """Solver.evolve_to_time raises "Maximum number of timesteps" only when all max_steps attempts
+    were accepted short of tend.  Here the dt_initial = 0.1 attempt is rejected, so 3 attempts give
+    2 steps: the loop ends short of tend without an error, in orc_evolve and the source loop alike;
+    with a dt that is never rejected the same max_steps is an error."""
+    q, dx, dy = problems.radial_dam_break(40, 40)
+    d = corc.make_desc("swe", 40, 40, dx, dy, grav=1.0)
+    c = corc.make_ctrl(max_steps=3)
+    for run in (corc.evolve, corc.evolve_with_source):
+        _, st = run(d, c, q, tend=1.0)
+        assert st["error"] == 0 and st["numsteps"] == 2 and st["rejected"] == 1 and st["t"] < 1.0
+        _, st = run(d, c, q, tend=1.0, dt=0.01)
+        assert st["error"] == 1 and st["numsteps"] == 3 and st["rejected"] == 0
