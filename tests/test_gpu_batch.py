@@ -87,15 +87,32 @@ def test_graph_replays_match_eager():
             assert np.array_equal(eager.download(g, ce), graphed.download(g, cg)), (r, g)
 
 
+def _gentle_quadrants(mx, my, seed, gamma=1.4):
+    """A four-quadrant Euler problem on [0,1]^2 with rho, p ~ U[0.5, 1.5] and u, v ~ U[-0.3, 0.3]
+    per quadrant: it stays positive next to walls (the faster quadrants of the bench workload
+    empty a wall's neighbourhood into near vacuum, where the Roe solver fails)."""
+    rng = np.random.default_rng(seed)
+    rho, p = rng.uniform(0.5, 1.5, 4), rng.uniform(0.5, 1.5, 4)
+    u, v = rng.uniform(-0.3, 0.3, 4), rng.uniform(-0.3, 0.3, 4)
+    xc, dx = problems.centers(0.0, 1.0, mx)
+    yc, dy = problems.centers(0.0, 1.0, my)
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    k = (X >= 0.5).astype(int) + 2 * (Y >= 0.5).astype(int)
+    q = np.zeros((4, mx, my), order="F")
+    q[0], q[1], q[2] = rho[k], rho[k] * u[k], rho[k] * v[k]
+    q[3] = p[k] / (gamma - 1.0) + 0.5 * rho[k] * (u[k] ** 2 + v[k] ** 2)
+    return q, dx, dy
+
+
 def _batch_method(solver, dtype, bc_lower, bc_upper, mthlim, ngrids=10, seed=11):
-    """Grids of uneven sizes for one method: Euler quadrants (seed 1, screened for positivity) or
-    16-hump shallow water, with the given BCs and per-wave limiters."""
+    """Grids of uneven sizes for one method: gentle Euler quadrants or 16-hump shallow water,
+    with the given BCs and per-wave limiters."""
     from clawpack_amd.batch import BatchedGrids
     from clawpack_amd.device import make_desc
     rng = np.random.default_rng(seed)
     sizes = rng.integers(24, 150, size=(ngrids, 2))
     if solver == "euler":
-        grids = [problems.euler_quadrants(int(mx), int(my), 1) for mx, my in sizes]
+        grids = [_gentle_quadrants(int(mx), int(my), 100 + g) for g, (mx, my) in enumerate(sizes)]
         sid, neq = 1, 4
     else:
         grids = [problems.gaussian_humps(int(mx), int(my), seed + g)
