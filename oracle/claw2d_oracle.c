@@ -53,6 +53,8 @@ typedef struct {
     int num_aux;          /* method(7); aux(1) is the bathymetry b for ORC_SWE_BATHY */
     int fwave;            /* solver.fwave: the Riemann solver returns f-waves Z (sum Z = flux jump) */
     double dry_tolerance; /* problem_data['dry_tolerance'] (ORC_SWE_BATHY) */
+    int mcapa;            /* method(6): 1-based aux index of the capacity kappa, 0 = none
+                             (pyclaw: state.index_capa + 1) */
 } orc_desc;
 
 typedef struct {
@@ -77,7 +79,8 @@ typedef struct {
 typedef struct {
     int cap;
     double *wave, *s, *amdq, *apdq, *cqxx, *bmasdq, *bpasdq, *qadd, *fadd, *gadd, *q1d;
-    double *aux1d; /* aux(maux, slice) */
+    double *aux1d;  /* aux(maux, slice) */
+    double *dtdx1d; /* dt/dx per slice cell: dtdx / kappa with a capacity array, else dtdx */
 } orc_work;
 
 static int work_alloc(orc_work *w, int nslice, int meqn, int mwaves, int maux) {
@@ -94,14 +97,15 @@ static int work_alloc(orc_work *w, int nslice, int meqn, int mwaves, int maux) {
     w->fadd = calloc((size_t)nslice * meqn, sizeof(double));
     w->gadd = calloc((size_t)nslice * meqn * 2, sizeof(double));
     w->q1d = calloc((size_t)nslice * meqn, sizeof(double));
+    w->dtdx1d = calloc((size_t)nslice, sizeof(double));
     return (w->wave && w->s && w->amdq && w->apdq && w->cqxx && w->bmasdq && w->bpasdq &&
-            w->qadd && w->fadd && w->gadd && w->q1d && w->aux1d) ? 0 : -1;
+            w->qadd && w->fadd && w->gadd && w->q1d && w->aux1d && w->dtdx1d) ? 0 : -1;
 }
 
 static void work_free(orc_work *w) {
     free(w->wave); free(w->s); free(w->amdq); free(w->apdq); free(w->cqxx);
     free(w->bmasdq); free(w->bpasdq); free(w->qadd); free(w->fadd); free(w->gadd); free(w->q1d);
-    free(w->aux1d);
+    free(w->aux1d); free(w->dtdx1d);
 }
 
 /* 1-based slice index -> 0-based storage offset */
@@ -111,6 +115,7 @@ static void work_free(orc_work *w) {
 #define SP(mw, i) s[(mw) + mwaves * SL(i)]
 #define A1(arr, m, i) arr[(m) + meqn * SL(i)]
 #define GADD(m, k, i) gadd[(m) + meqn * ((k) + 2 * SL(i))]
+#define DTDX(i) dtdx1d[SL(i)]
 
 /* ------------------------------------------------------------------------------------------
  * rpn2, shallow water, Roe linearisation + Harten-Hyman entropy fix.
@@ -524,8 +529,11 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
     const int g = d->num_ghost, meqn = d->num_eqn, mwaves = d->num_waves;
     double *q1d = w->q1d, *wave = w->wave, *s = w->s, *amdq = w->amdq, *apdq = w->apdq;
     double *cqxx = w->cqxx, *bmasdq = w->bmasdq, *bpasdq = w->bpasdq;
-    double *qadd = w->qadd, *fadd = w->fadd, *gadd = w->gadd;
+    double *qadd = w->qadd, *fadd = w->fadd, *gadd = w->gadd, *dtdx1d = w->dtdx1d;
     const int len = n + 2 * g;
+    /* step2.f90: dtdx1d = dtdx / aux(mcapa, slice) with a capacity array, else dtdx */
+    for (int k = 0; k < len; ++k)
+        dtdx1d[k] = d->mcapa > 0 ? dtdx / w->aux1d[(d->mcapa - 1) + d->num_aux * k] : dtdx;
     memset(qadd, 0, sizeof(double) * meqn * len);
     memset(fadd, 0, sizeof(double) * meqn * len);
     memset(gadd, 0, sizeof(double) * meqn * 2 * len);
@@ -541,15 +549,17 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
 
     for (int i = 1; i <= n + 1; ++i)
         for (int m = 0; m < meqn; ++m) {
-            A1(qadd, m, i) -= dtdx * A1(apdq, m, i);
-            A1(qadd, m, i - 1) -= dtdx * A1(amdq, m, i);
+            A1(qadd, m, i) -= DTDX(i) * A1(apdq, m, i);
+            A1(qadd, m, i - 1) -= DTDX(i - 1) * A1(amdq, m, i);
         }
 
+    /* a right-going wave is measured against the right cell's capacity, a left-going one
+     * against the left cell's */
     double c = 0.0;
     for (int i = 1; i <= n + 1; ++i)
         for (int mw = 0; mw < mwaves; ++mw) {
-            c = fmax(c, dtdx * SP(mw, i));
-            c = fmax(c, -dtdx * SP(mw, i));
+            c = fmax(c, DTDX(i) * SP(mw, i));
+            c = fmax(c, -DTDX(i - 1) * SP(mw, i));
         }
     *cfl1d = c;
 
@@ -559,7 +569,9 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
             if (d->mthlim[mw] > 0) limit = 1;
         if (limit) limiter(n, g, meqn, mwaves, wave, s, d->mthlim);
         for (int i = 1; i <= n + 1; ++i) {
-            const double dtdxave = 0.5 * (dtdx + dtdx);
+            /* the average of dtdx1d over the two cells, in the correction only (not in the
+             * transverse terms) */
+            const double dtdxave = 0.5 * (DTDX(i - 1) + DTDX(i));
             for (int m = 0; m < meqn; ++m) {
                 double cq = 0.0;
                 for (int mw = 0; mw < mwaves; ++mw) {
@@ -588,8 +600,8 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
         rpt2_euler(ixy, n, g, meqn, q1d, d->gamma, amdq, bmasdq, bpasdq);
     for (int i = 1; i <= n + 1; ++i)
         for (int m = 0; m < meqn; ++m) {
-            GADD(m, 0, i - 1) -= 0.5 * dtdx * A1(bmasdq, m, i);
-            GADD(m, 1, i - 1) -= 0.5 * dtdx * A1(bpasdq, m, i);
+            GADD(m, 0, i - 1) -= 0.5 * DTDX(i - 1) * A1(bmasdq, m, i);
+            GADD(m, 1, i - 1) -= 0.5 * DTDX(i - 1) * A1(bpasdq, m, i);
         }
     if (d->solver == ORC_SWE)
         rpt2_swe(ixy, n, g, meqn, q1d, d->grav, apdq, bmasdq, bpasdq);
@@ -599,8 +611,8 @@ static void flux2(const orc_desc *d, int ixy, int n, double dtdx, orc_work *w, d
         rpt2_euler(ixy, n, g, meqn, q1d, d->gamma, apdq, bmasdq, bpasdq);
     for (int i = 1; i <= n + 1; ++i)
         for (int m = 0; m < meqn; ++m) {
-            GADD(m, 0, i) -= 0.5 * dtdx * A1(bmasdq, m, i);
-            GADD(m, 1, i) -= 0.5 * dtdx * A1(bpasdq, m, i);
+            GADD(m, 0, i) -= 0.5 * DTDX(i) * A1(bmasdq, m, i);
+            GADD(m, 1, i) -= 0.5 * DTDX(i) * A1(bpasdq, m, i);
         }
 }
 
@@ -630,6 +642,19 @@ static void xsweep_row(const orc_desc *d, int j, const double *qold, const doubl
     flux2(d, 1, mx, dtdx, &w, &cfl1d);
     *cfl = fmax(*cfl, cfl1d);
     const double *qadd = w.qadd, *fadd = w.fadd, *gadd = w.gadd;
+    if (d->mcapa > 0) {
+        /* capacity form: the flux differences of a cell are divided by its kappa */
+        const int kc = d->mcapa - 1;
+        for (int i = 1; i <= mx; ++i)
+            for (int m = 0; m < meqn; ++m) {
+                QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, i)
+                                    - (dtdx * (A1(fadd, m, i + 1) - A1(fadd, m, i))
+                                       + dtdy * (GADD(m, 1, i) - GADD(m, 0, i))) / AB(auxbc, kc, i, j);
+                QB(qnew, m, i, j - 1) = QB(qnew, m, i, j - 1) - dtdy * GADD(m, 0, i) / AB(auxbc, kc, i, j - 1);
+                QB(qnew, m, i, j + 1) = QB(qnew, m, i, j + 1) + dtdy * GADD(m, 1, i) / AB(auxbc, kc, i, j + 1);
+            }
+        return;
+    }
     for (int i = 1; i <= mx; ++i)
         for (int m = 0; m < meqn; ++m) {
             QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, i)
@@ -654,6 +679,18 @@ static void ysweep_col(const orc_desc *d, int i, const double *qold, const doubl
     flux2(d, 2, my, dtdy, &w, &cfl1d);
     *cfl = fmax(*cfl, cfl1d);
     const double *qadd = w.qadd, *fadd = w.fadd, *gadd = w.gadd;
+    if (d->mcapa > 0) {
+        const int kc = d->mcapa - 1;
+        for (int j = 1; j <= my; ++j)
+            for (int m = 0; m < meqn; ++m) {
+                QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, j)
+                                    - (dtdy * (A1(fadd, m, j + 1) - A1(fadd, m, j))
+                                       + dtdx * (GADD(m, 1, j) - GADD(m, 0, j))) / AB(auxbc, kc, i, j);
+                QB(qnew, m, i - 1, j) = QB(qnew, m, i - 1, j) - dtdx * GADD(m, 0, j) / AB(auxbc, kc, i - 1, j);
+                QB(qnew, m, i + 1, j) = QB(qnew, m, i + 1, j) + dtdx * GADD(m, 1, j) / AB(auxbc, kc, i + 1, j);
+            }
+        return;
+    }
     for (int j = 1; j <= my; ++j)
         for (int m = 0; m < meqn; ++m) {
             QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, j)
@@ -834,7 +871,9 @@ int orc_step2ds_aux(const orc_desc *d, const double *qold, const double *auxbc, 
             for (int i = 1; i <= mx; ++i)
                 for (int m = 0; m < meqn; ++m)
                     QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, i)
-                                        - dtdx * (A1(fadd, m, i + 1) - A1(fadd, m, i));
+                                        - (d->mcapa > 0 ? dtdx * (A1(fadd, m, i + 1) - A1(fadd, m, i))
+                                                              / AB(auxbc, d->mcapa - 1, i, j)
+                                                        : dtdx * (A1(fadd, m, i + 1) - A1(fadd, m, i)));
         }
     } else {
         const double dtdy = dt / d->dy;
@@ -850,7 +889,9 @@ int orc_step2ds_aux(const orc_desc *d, const double *qold, const double *auxbc, 
             for (int j = 1; j <= my; ++j)
                 for (int m = 0; m < meqn; ++m)
                     QB(qnew, m, i, j) = QB(qnew, m, i, j) + A1(qadd, m, j)
-                                        - dtdy * (A1(fadd, m, j + 1) - A1(fadd, m, j));
+                                        - (d->mcapa > 0 ? dtdy * (A1(fadd, m, j + 1) - A1(fadd, m, j))
+                                                              / AB(auxbc, d->mcapa - 1, i, j)
+                                                        : dtdy * (A1(fadd, m, j + 1) - A1(fadd, m, j)));
         }
     }
     *cfl_out = cfl;

@@ -23,6 +23,7 @@ class Desc(C.Structure):
         ("transverse_waves", C.c_int), ("mthlim", C.c_int * 4), ("dx", C.c_double),
         ("dy", C.c_double), ("grav", C.c_double), ("gamma", C.c_double),
         ("num_aux", C.c_int), ("fwave", C.c_int), ("dry_tolerance", C.c_double),
+        ("mcapa", C.c_int),
     ]
 
 
@@ -117,9 +118,12 @@ def _solver_id(solver):
 
 
 def make_desc(solver, mx, my, dx, dy, grav=1.0, gamma=1.4, order=2, transverse_waves=2,
-              mthlim=(4, 4, 4, 4), num_ghost=2, num_aux=None, dry_tolerance=1e-3, fwave=None):
+              mthlim=(4, 4, 4, 4), num_ghost=2, num_aux=None, dry_tolerance=1e-3, fwave=None,
+              index_capa=-1):
     """fwave and num_aux default to what the solver needs (f-waves and one aux field, the
-    bathymetry, for shallow_bathymetry_fwave_2D)."""
+    bathymetry, for shallow_bathymetry_fwave_2D).  index_capa >= 0: aux[index_capa] is the
+    capacity function kappa (pyclaw's state.index_capa; the step divides each cell's flux
+    differences by its kappa)."""
     d = Desc()
     d.solver = _solver_id(solver)
     d.num_eqn = 4 if d.solver == EULER else 3
@@ -133,6 +137,9 @@ def make_desc(solver, mx, my, dx, dy, grav=1.0, gamma=1.4, order=2, transverse_w
     for k in range(4):
         d.mthlim[k] = int(lim[k])
     d.dx, d.dy, d.grav, d.gamma = dx, dy, grav, gamma
+    if index_capa >= d.num_aux:
+        raise ValueError("index_capa %d needs num_aux > %d" % (index_capa, index_capa))
+    d.mcapa = int(index_capa) + 1
     return d
 
 
