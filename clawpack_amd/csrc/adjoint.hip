@@ -258,6 +258,7 @@ static int check_adjoint_desc(const claw2d_desc* d) {
   if (d->solver != CLAW_SOLVER_SHALLOW_ROE_EFIX)
     return set_error(-20, "the adjoint step is implemented for shallow water (shallow_roe_with_efix) only");
   if (d->dtype != CLAW_FP64) return set_error(-21, "the adjoint step runs in FP64 only");
+  if (d->num_aux != 0) return set_error(-23, "the adjoint step takes no aux arrays (no capacity function)");
   return 0;
 }
 

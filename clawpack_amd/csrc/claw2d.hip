@@ -167,9 +167,11 @@ template <class T, class P> struct StepArgs {
   int fuse;
   int total_waves;
   double cfl_max, cfl_desired, dt_max;
-  // aux plane 0 (the bathymetry of an f-wave policy) and its boundary conditions
+  // aux plane 0 (the bathymetry of an f-wave policy) and the aux boundary conditions
   const T* aux;
   int abx_lo, abx_hi, aby_lo, aby_hi;
+  // the capacity plane kappa (aux plane mcapa-1) of a capacity-form step, else null
+  const T* capa;
   typename P::Par par;
 };
 
@@ -191,19 +193,26 @@ __device__ __forceinline__ typename P::Cell prep_cell(const T q[P::NEQ], T b, co
 // the eigenvector products e_p(neighbour).e_p(this) the limiter needs.
 // TWC >= 0: the method is known at compile time (order 2, transverse_waves = TWC), so the
 // order/transverse selects compile away; TWC < 0: order and transverse_waves from a.
-template <class T, class P, int LIM, int TWC>
+// CAPA (capacity form, step2.f90/flux2.f90 with mcapa > 0): rkL/rkR are 1/kappa of the left and
+// right cells.  Every contribution formed here is later scaled by 1/kappa of the cell that
+// receives it (the capacity-form update divides a cell's flux differences by its kappa and qadd
+// carries dt/(dx kappa) of the cell it updates); in addition the correction uses the average
+// (dt/dx)(1/kappa_L + 1/kappa_R)/2 in place of dt/dx, and the transverse terms of amdq (apdq)
+// carry dt/(dx kappa) of the left (right) cell.
+template <class T, class P, int LIM, int TWC, bool CAPA = false>
 __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T alL[P::NW],
                                              const T mL[P::NW], const T alR[P::NW],
                                              const T mR[P::NW], T r, T rt,
                                              const StepArgs<T, P>& a, T LX[P::NEQ], T LG1[P::NEQ],
                                              T LG2[P::NEQ], T RX[P::NEQ], T RG1[P::NEQ],
-                                             T RG2[P::NEQ]) {
+                                             T RG2[P::NEQ], T rkL = T(1), T rkR = T(1)) {
   constexpr int NEQ = P::NEQ, NW = P::NW;
   const bool second = TWC >= 0 ? true : a.order != 1;
   const int tw = TWC >= 0 ? TWC : a.tw;
   const bool fold = second && tw == 2;
   T in[NW];
   if (second) P::inv_norm(f, in);
+  const T rc = CAPA ? T(0.5) * (r * rkL + r * rkR) : r;  // dt/dx of the correction
   T XL[NW], XR[NW], A[NW], B[NW];
 #pragma unroll
   for (int p = 0; p < NW; ++p) {
@@ -216,7 +225,7 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
       const int meth = LIM > 0 ? LIM : a.mthlim[p];
       const T sa = vabs(s);  // |s| as a source modifier (a select form costs two instructions)
       const T sw = P::FWAVE ? copysign(T(1), s) : sa;
-      k = sw * (T(1) - sa * r) * limited(al, t, meth);
+      k = sw * (T(1) - sa * rc) * limited(al, t, meth);
     }
     XL[p] = -r * fma(T(0.5), k, e);
     XR[p] = -r * fma(T(-0.5), k, q);
@@ -232,24 +241,30 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
     P::tbeta(f, A, bA);
     P::tbeta(f, B, bB);
     const T w = T(-0.5) * r * rt;
+    const T wA = CAPA ? T(-0.5) * (r * rkL) * rt : w, wB = CAPA ? T(-0.5) * (r * rkR) * rt : w;
     // transversely subsonic on every lane (v - a < 0 < v + a): the down/up split has a fixed
     // pattern, so the zero entries are skipped; the result is the same bit for bit, because both
     // forms take their interface constants from the one tc formed here
     const typename P::TConst tc = P::tconst(f);
     const bool sub = tc.vm < T(0) && tc.vp > T(0);
     if (__ballot(!sub) == 0ull) {
-      P::expand_t_split(f, tc, w, bA, LG1, LG2);
-      P::expand_t_split(f, tc, w, bB, RG1, RG2);
+      P::expand_t_split(f, tc, wA, bA, LG1, LG2);
+      P::expand_t_split(f, tc, wB, bB, RG1, RG2);
     } else {
       T sg[NW], c1[NW], c2[NW], c3[NW], c4[NW];
       P::tspeeds(f, tc, sg);
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
-        const T wm = w * vmin(sg[j], T(0)), wp = w * vmax(sg[j], T(0));
+        const T wm = wA * vmin(sg[j], T(0)), wp = wA * vmax(sg[j], T(0));
         c1[j] = wm * bA[j];
         c2[j] = wp * bA[j];
-        c3[j] = wm * bB[j];
-        c4[j] = wp * bB[j];
+        if constexpr (CAPA) {
+          c3[j] = (wB * vmin(sg[j], T(0))) * bB[j];
+          c4[j] = (wB * vmax(sg[j], T(0))) * bB[j];
+        } else {
+          c3[j] = wm * bB[j];
+          c4[j] = wp * bB[j];
+        }
       }
       P::expand_t(f, tc, c1, LG1);
       P::expand_t(f, tc, c2, LG2);
@@ -275,12 +290,18 @@ template <int NEQ, class T> __device__ __forceinline__ void yswap(T v[NEQ]) {
 // SWEEP (split, padded): 0 both sweeps in one pass; 1 the x-sweep only, q* written into the
 // interior of the padded qout (its ghost cells are left to the caller); 2 the y-sweep only, from a
 // padded q* into the interior planes qout.
-template <class T, class P, int LIM, bool SPLIT = false, int TWC = -1, bool PAD = false, int SWEEP = 0>
+// CAPA: capacity form with kappa = a.capa (see finish_iface).  The accumulators hold each cell's
+// update before its 1/kappa, applied when the cell is written.  For the split step the x-sweep of
+// a ghost row takes kappa from the row its state comes from, so its q* equals the ghost fill of
+// the x-swept interior (the y-sweep sees the aux ghost fill of kappa, as the reference's slice).
+template <class T, class P, int LIM, bool SPLIT = false, int TWC = -1, bool PAD = false, int SWEEP = 0,
+          bool CAPA = false>
 __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __restrict__ qin,
                                            T* __restrict__ qout, const int mx, const int my,
                                            const int x0, const int y0, const int y1, const double dt,
                                            const double dx, const double dy, claw2d_ctrl* ctrl) {
   static_assert(SWEEP == 0 || (SPLIT && PAD), "single sweeps run on the padded split path");
+  static_assert(!(CAPA && PAD), "the capacity form runs on the index-mapped (unpadded) path");
   constexpr bool DPP = kDppShift<T, P>;
   // zero-filled edges pay for the Euler kernels (FP32 -4%) but not for shallow water (FP64 +0.7%)
   constexpr bool DPPZ = P::NEQ == 4;
@@ -298,7 +319,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   if constexpr (PAD) cx = c < -2 ? -2 : (c > mx + 1 ? mx + 1 : c);  // lanes past the halo: unused data
   else map_index(c, mx, a.bcx_lo, a.bcx_hi, cx, fx);
   int cxa = cx;  // aux column (aux BCs may differ from the state's)
-  if constexpr (P::NAUX > 0) {
+  if constexpr (P::NAUX > 0 || CAPA) {
     bool fa;
     map_index(c, mx, a.abx_lo, a.abx_hi, cxa, fa);
   }
@@ -313,7 +334,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   const ptrdiff_t ldq = PAD ? (ptrdiff_t)mx + 4 : (ptrdiff_t)mx;  // row stride of qin
   const size_t plane_in = PAD ? (size_t)(mx + 4) * (size_t)(my + 4) : plane;
   const T* const qin0 = PAD ? qin + 2 * ldq + 2 : qin;             // cell (0, 0)
-  auto load_row = [&](int R, T q[NEQ], bool& fy, T& b) {
+  auto load_row = [&](int R, T q[NEQ], bool& fy, T& b, T& kap, T& kapx) {
     int ry = R;
     fy = false;
     if constexpr (!PAD) {
@@ -330,6 +351,15 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       ra = __builtin_amdgcn_readfirstlane(ra);
       b = a.aux[(size_t)ra * (size_t)mx + (size_t)cxa];
     }
+    if constexpr (CAPA) {
+      int ra = R;
+      bool fa;
+      if (R < 0 || R >= my) map_index(R, my, a.aby_lo, a.aby_hi, ra, fa);
+      ra = __builtin_amdgcn_readfirstlane(ra);
+      kap = a.capa[(size_t)ra * (size_t)mx + (size_t)cxa];
+      if constexpr (SPLIT) kapx = a.capa[(size_t)ry * (size_t)mx + (size_t)cxa];
+      else kapx = kap;
+    }
   };
   auto apply_flips = [&](T q[NEQ], bool fy) {
     if (!walls) return;
@@ -339,6 +369,15 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       q[m] = neg ? -q[m] : q[m];
     }
   };
+
+  // CFL speed of an interface: max |s| / kappa over its waves, a right-going wave against the
+  // right cell's kappa and a left-going one against the left cell's (flux2.f90's cfl1d with
+  // dtdx1d); every policy's extreme speeds are u - a and u + a
+  auto cfl_speed = [&](const Iface& f, T rkl, T rkr) -> T {
+    if constexpr (CAPA) return vmax((f.u + f.a) * rkr, (f.a - f.u) * rkl);
+    else return P::smax(f);
+  };
+  T rkprev = T(1), rkm2 = T(1);  // 1/kappa of rows R-1 and R-2 (capacity form)
 
   T smx = T(0), smy = T(0);
   // FP64: plain max plus a NaN flag folded in once at the end (the same result as a
@@ -364,8 +403,8 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   // the x-sweep alone needs only the owned rows; every other form marches two halo rows each side
   const int rlo = SWEEP == 1 ? y0 : y0 - 2, rhi = SWEEP == 1 ? y1 - 1 : y1 + 1;
   bool fnext;
-  T bnext = T(0);
-  load_row(rlo, qnext, fnext, bnext);
+  T bnext = T(0), knext = T(1), kxnext = T(1);
+  load_row(rlo, qnext, fnext, bnext, knext, kxnext);
   // One marched row.  ST (steady): a row y0+2..y1-1 of the full step, for which every row test
   // below holds, so that copy of the body is one straight block the scheduler can interleave
   // (shallow water: 2.4% faster in FP64 on MI355X; the Euler kernel would spill, so it keeps one
@@ -376,8 +415,14 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
 #pragma unroll
     for (int m = 0; m < NEQ; ++m) qR[m] = qnext[m];
     const T bR = bnext;
+    // 1/kappa of row R as the y-sweeps see it (rkR) and as its x-sweep sees it (rkX)
+    T rkR = T(1), rkX = T(1);
+    if constexpr (CAPA) {
+      rkR = frcp(knext);
+      rkX = SPLIT ? frcp(kxnext) : rkR;
+    }
     apply_flips(qR, fnext);
-    if (ST || R < rhi) load_row(R + 1, qnext, fnext, bnext);
+    if (ST || R < rhi) load_row(R + 1, qnext, fnext, bnext, knext, kxnext);
     const Cell cR = prep_cell<P>(qR, bR, a.par);
 
     // ---- request the left neighbour's cell data for the x-interfaces of row R now, so that the
@@ -389,6 +434,8 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
 #pragma unroll
       for (int k = 0; k < NC; ++k) pl[k] = from_left<DPP, T, DPPZ>(pk[k]);
     }
+    T rkLx = T(1);  // 1/kappa of the left cell of the row's x-interface
+    if constexpr (CAPA && SWEEP != 2) rkLx = from_left<DPP, T, DPPZ>(rkX);
 
     // ---- y-sweep part of row R: the y-interface between rows R-1 and R (in-lane), then the
     // y-interface one row lower, whose limiter needed it.  cY is the cell data the y-sweeps see:
@@ -403,7 +450,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         // in an accepted state is still caught by the stored-cell check)
         if ((SPLIT ? own_col : ycfl_col) && (ST || (R >= 0 && R <= my)))
         {
-          const T sp = P::smax(fynew);
+          const T sp = cfl_speed(fynew, rkprev, rkR);
           if constexpr (SPLIT) smy = vmax(smy, sp);
           else if constexpr (kNanFlag) { smy = vmax(smy, sp); nany |= sp != sp; }
           else smy = nmax(smy, sp);
@@ -418,8 +465,8 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         P::mdot(lp, ln, mR);
         if (ST || R >= y0 + 1) {
           T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-          finish_iface<T, P, LIM, TWC>(fyprev, yal2, ymprev, fynew.al, mR, dtdy, dtdx, a, LX, LG1, LG2,
-                                  RX, RG1, RG2);
+          finish_iface<T, P, LIM, TWC, CAPA>(fyprev, yal2, ymprev, fynew.al, mR, dtdy, dtdx, a, LX, LG1,
+                                             LG2, RX, RG1, RG2, rkm2, rkprev);
           yswap<NEQ>(LX); yswap<NEQ>(LG1); yswap<NEQ>(LG2);
           yswap<NEQ>(RX); yswap<NEQ>(RG1); yswap<NEQ>(RG2);
           if constexpr (kImmY) {
@@ -450,7 +497,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
               const size_t off = (size_t)(R - 2) * (size_t)mx + (size_t)c;
   #pragma unroll
               for (int m = 0; m < NEQ; ++m) {
-                qn[m] = qm2[m] + acc0[m];
+                qn[m] = CAPA ? qm2[m] + acc0[m] * rkm2 : qm2[m] + acc0[m];
                 qout[(size_t)m * plane + off] = qn[m];
               }
               bad |= P::bad(qn);
@@ -484,7 +531,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       P::template rpn2<0>(cL, cR, a.par, fxi);
       if (xcfl_col && (ST || (SPLIT ? (R >= 0 && R < my) : (R >= -1 && R <= my))))
       {
-        const T sp = P::smax(fxi);
+        const T sp = cfl_speed(fxi, rkLx, rkX);
         if constexpr (kNanFlag) { smx = vmax(smx, sp); nanx |= sp != sp; }
         else smx = nmax(smx, sp);
       }
@@ -506,12 +553,16 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         alL[p] = lleft[p];
       }
       T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-      finish_iface<T, P, LIM, TWC>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2);
+      finish_iface<T, P, LIM, TWC, CAPA>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2,
+                                         rkLx, rkX);
       if constexpr (SPLIT) {
         // q* of row R: the x-sweep only (no transverse terms); valid on the owned columns
         T qs[NEQ];
 #pragma unroll
-        for (int m = 0; m < NEQ; ++m) qs[m] = qR[m] + (RX[m] + from_right<DPP, T, DPPZ>(LX[m]));
+        for (int m = 0; m < NEQ; ++m) {
+          const T dq = RX[m] + from_right<DPP, T, DPPZ>(LX[m]);
+          qs[m] = CAPA ? qR[m] + dq * rkX : qR[m] + dq;
+        }
         if constexpr (SWEEP == 1) {
           if (own_col) {
             const size_t off = (size_t)(R + 2) * (size_t)(mx + 4) + (size_t)(c + 2);
@@ -547,6 +598,8 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       }
       cprev = cY;
       fyprev = fynew;
+      rkm2 = rkprev;
+      rkprev = rkR;
     }
   };
   if constexpr (SWEEP == 0 && NEQ == 3) {
@@ -678,7 +731,7 @@ template <class T> struct MinWaves {
   static constexpr int value = sizeof(T) == 8 ? CLAW_MIN_WAVES_F64 : CLAW_MIN_WAVES_F32;
 };
 
-template <class T, class P, int LIM, bool SPLIT, int TWC, bool PAD = false, int SWEEP = 0>
+template <class T, class P, int LIM, bool SPLIT, int TWC, bool PAD = false, int SWEEP = 0, bool CAPA = false>
 __global__ void __launch_bounds__(kLanes* kWPB) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::value)))
 step2_fused(StepArgs<T, P> a) {
   double dt = a.dt_fixed;
@@ -695,7 +748,8 @@ step2_fused(StepArgs<T, P> a) {
   const int strip = wave % a.nstrips, seg = wave / a.nstrips;
   const int x0 = strip * kOwn, y0 = seg * a.seg_h;
   const int y1 = min(y0 + a.seg_h, a.my);
-  march_item<T, P, LIM, SPLIT, TWC, PAD, SWEEP>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy, a.ctrl);
+  march_item<T, P, LIM, SPLIT, TWC, PAD, SWEEP, CAPA>(a, qin, qout, a.mx, a.my, x0, y0, y1, dt, a.dx, a.dy,
+                                                      a.ctrl);
   if (a.fuse) {
     // The last wavefront to finish closes the step: its speed maxima are then complete.  The
     // wait for this wavefront's outstanding memory operations (its max/or atomics are acknowledged
@@ -910,10 +964,18 @@ static int check_desc(const claw2d_desc* d) {
   if (!(d->dx > 0) || !(d->dy > 0)) return fail(-13, "dx, dy must be positive");
   if (d->seg_h < 0) return fail(-14, "seg_h must be >= 0");
   if ((long long)d->mx * d->my > (1LL << 31)) return fail(-15, "grid too large for 32-bit row index math");
+  if (d->mcapa < 0 || d->mcapa > d->num_aux)
+    return fail(-29, "mcapa must be 0 (no capacity function) or a 1-based aux plane <= num_aux");
   if (d->solver == CLAW_SOLVER_SHALLOW_BATHY_FWAVE) {
     if (d->num_aux < 1) return fail(-16, "shallow_bathymetry_fwave needs num_aux >= 1 (aux plane 0 = bathymetry)");
     if (d->fwave != 1) return fail(-17, "shallow_bathymetry_fwave returns f-waves: fwave must be 1");
     if (!(d->dry_tolerance >= 0)) return fail(-18, "dry_tolerance must be >= 0");
+  } else {
+    if (d->num_aux != 0 && d->mcapa == 0)
+      return fail(-16, "this Riemann solver reads no aux arrays: num_aux > 0 needs a capacity function (mcapa)");
+    if (d->fwave != 0) return fail(-17, "fwave = 1 needs an f-wave Riemann solver (shallow_bathymetry_fwave)");
+  }
+  if (d->num_aux > 0) {
     for (int k = 0; k < 2; ++k) {
       const int lo = d->aux_bc_lower[k], hi = d->aux_bc_upper[k];
       if (lo < 1 || lo > 3 || hi < 1 || hi > 3)
@@ -921,9 +983,6 @@ static int check_desc(const claw2d_desc* d) {
       if ((lo == CLAW_BC_PERIODIC) != (hi == CLAW_BC_PERIODIC))
         return fail(-11, "periodic aux bc must be set on both sides");
     }
-  } else {
-    if (d->num_aux != 0) return fail(-16, "this Riemann solver takes no aux arrays (num_aux must be 0)");
-    if (d->fwave != 0) return fail(-17, "fwave = 1 needs an f-wave Riemann solver (shallow_bathymetry_fwave)");
   }
   return 0;
 }
@@ -947,6 +1006,7 @@ static void fill_args(StepArgs<T, P>& a, const claw2d_desc* d, long resident) {
   a.tw = d->transverse_waves;
   for (int p = 0; p < 4; ++p) a.mthlim[p] = d->mthlim[p];
   a.aux = nullptr;
+  a.capa = nullptr;
   a.abx_lo = d->aux_bc_lower[0];
   a.abx_hi = d->aux_bc_upper[0];
   a.aby_lo = d->aux_bc_lower[1];
@@ -1007,8 +1067,13 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
                        bool pad = false) {
   int slot = 5;
   const void* kern = (const void*)step2_fused<T, P, 0, false, -1, true>;
-  if (!pad || SPLIT) kern = step_kernel<T, P, SPLIT>(d, &slot);
-  static thread_local long cached[6] = {-1, -1, -1, -1, -1, -1};
+  if (d->mcapa > 0) {
+    slot = 6;  // the capacity form runs the general kernel
+    kern = (const void*)step2_fused<T, P, 0, SPLIT, -1, false, 0, true>;
+  } else if (!pad || SPLIT) {
+    kern = step_kernel<T, P, SPLIT>(d, &slot);
+  }
+  static thread_local long cached[7] = {-1, -1, -1, -1, -1, -1, -1};
   if (cached[slot] < 0) cached[slot] = resident_waves(kern);
   StepArgs<T, P> a;
   fill_args(a, d, cached[slot]);
@@ -1024,6 +1089,7 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   a.dt_max = d->dt_max;
   if (SPLIT) a.tw = 0;  // dimensional splitting propagates no transverse waves (method(3) < 0)
   a.aux = (const T*)aux;
+  if (d->mcapa > 0) a.capa = (const T*)aux + (size_t)(d->mcapa - 1) * (size_t)d->mx * (size_t)d->my;
   const int waves = a.nstrips * a.nsegs;
   a.total_waves = waves;
   const int blocks = (waves + kWPB - 1) / kWPB;
@@ -1171,7 +1237,7 @@ static int check_batch(const claw2d_desc* d, const void* grids, int32_t ngrids, 
                        const void* q0, const void* q1, const void* ctrls, const void* counter) {
   int rc = check_desc(d);
   if (rc) return rc;
-  if (d->num_aux > 0) return fail(-25, "batched grids take no aux arrays");
+  if (d->num_aux > 0) return fail(-25, "batched grids take no aux arrays (no capacity function)");
   if (!grids || !q0 || !q1 || !ctrls || !counter) return fail(-22, "null buffer");
   if (q0 == q1) return fail(-23, "q0 and q1 must differ");
   if (ngrids < 1) return fail(-30, "ngrids must be >= 1");
@@ -1288,6 +1354,7 @@ static int padded_desc(const claw2d_desc* d, claw2d_desc* dd) {
     if (lo < 0 || lo > 3 || hi < 0 || hi > 3) return fail(-10, "bc must be custom(0), extrap(1), periodic(2) or wall(3)");
     dd->bc_lower[k] = dd->bc_upper[k] = CLAW_BC_EXTRAP;
   }
+  if (dd->mcapa != 0) return fail(-28, "the padded (custom-BC) steps take no capacity function (mcapa must be 0)");
   return check_desc(dd);
 }
 }  // namespace claw

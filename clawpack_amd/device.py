@@ -24,12 +24,17 @@ def make_desc(solver_id, dtype, mx, my, dx, dy, num_eqn, num_waves, order=2, tra
               mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_upper=(1, 1), reflect_index=(1, 2),
               grav=1.0, gamma=1.4, cfl_max=1.0, cfl_desired=0.9, dt_max=1e99, dt_variable=True,
               seg_h=0, num_ghost=2, num_aux=None, fwave=None, aux_bc_lower=None, aux_bc_upper=None,
-              dry_tolerance=1e-3):
+              dry_tolerance=1e-3, index_capa=-1):
     """num_aux and fwave default to what the solver needs (one aux field, the bathymetry, and
-    f-waves for shallow_bathymetry_fwave); aux BCs default to the state's."""
+    f-waves for shallow_bathymetry_fwave; with a capacity function at least index_capa + 1 aux
+    fields); aux BCs default to the state's.  index_capa >= 0: aux[index_capa] is the capacity
+    function kappa (pyclaw's state.index_capa), the step is the capacity form."""
     d = ClawDesc()
     bathy = int(solver_id) == _lib.SOLVER_SHALLOW_BATHY_FWAVE
-    d.num_aux = int((1 if bathy else 0) if num_aux is None else num_aux)
+    if num_aux is None:
+        num_aux = max(1 if bathy else 0, int(index_capa) + 1)
+    d.num_aux = int(num_aux)
+    d.mcapa = int(index_capa) + 1 if index_capa is not None and index_capa >= 0 else 0
     d.fwave = int(bool(bathy if fwave is None else fwave))
     d.dry_tolerance = float(dry_tolerance)
     abl = bc_lower if aux_bc_lower is None else aux_bc_lower

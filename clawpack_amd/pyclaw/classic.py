@@ -121,11 +121,18 @@ class ClawSolver2D(Solver):
             raise ValueError("solver.fwave must be %s for %s (its Riemann solver returns %s)"
                              % (rp_fwave, pol.name, "f-waves" if rp_fwave else "waves"))
         rp_aux = pol.num_aux
-        if state.index_capa >= 0:
-            raise NotImplementedError("a capacity function (index_capa) is not supported on the HIP path")
-        if rp_aux == 0 and state.num_aux > 0:
-            raise NotImplementedError("%s reads no aux arrays; aux is only supported with "
-                                      "shallow_bathymetry_fwave_2D on the HIP path" % pol.name)
+        capa = state.index_capa >= 0
+        if capa:
+            if state.index_capa >= state.num_aux:
+                raise ValueError("state.index_capa = %d needs num_aux > %d (aux[index_capa] is the "
+                                 "capacity function)" % (state.index_capa, state.index_capa))
+            if self._has_custom_bc():
+                raise NotImplementedError("a capacity function (index_capa) with BC.custom is not "
+                                          "supported on the HIP path")
+        if rp_aux == 0 and state.num_aux > 0 and not capa:
+            raise NotImplementedError("%s reads no aux arrays; aux is supported with "
+                                      "shallow_bathymetry_fwave_2D or a capacity function "
+                                      "(state.index_capa) on the HIP path" % pol.name)
         if state.num_aux < rp_aux:
             raise ValueError("%s needs num_aux >= %d (aux[0] = bathymetry)" % (pol.name, rp_aux))
         if state.num_aux > 0:
@@ -197,7 +204,7 @@ class ClawSolver2D(Solver):
             seg_h=self.seg_h, num_aux=state.num_aux, fwave=bool(self.fwave),
             aux_bc_lower=[b if b is not None else 1 for b in self.aux_bc_lower],
             aux_bc_upper=[b if b is not None else 1 for b in self.aux_bc_upper],
-            dry_tolerance=pd.get("dry_tolerance", 0.0))
+            dry_tolerance=pd.get("dry_tolerance", 0.0), index_capa=state.index_capa)
 
     def setup(self, solution):
         state = solution.state

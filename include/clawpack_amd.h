@@ -69,15 +69,25 @@ typedef struct claw2d_desc {
   double grav;              /* problem_data['grav'] (shallow water) */
   double gamma;             /* problem_data['gamma'] (Euler) */
   double cfl_max, cfl_desired, dt_max;
-  /* aux arrays (method(7) = num_aux) and f-waves (solver.fwave).  Both are required by, and only
-   * accepted with, CLAW_SOLVER_SHALLOW_BATHY_FWAVE: num_aux >= 1 (aux plane 0 is the bathymetry
-   * b), fwave = 1.  aux_bc_* are solver.aux_bc_lower/upper (extrap, periodic or wall; a wall
-   * mirrors aux without a sign change). */
+  /* aux arrays (method(7) = num_aux) and f-waves (solver.fwave).  CLAW_SOLVER_SHALLOW_BATHY_FWAVE
+   * requires num_aux >= 1 (aux plane 0 is the bathymetry b) and fwave = 1; the other solvers take
+   * aux arrays only with a capacity function (mcapa below) and fwave = 0.  aux_bc_* are
+   * solver.aux_bc_lower/upper (extrap, periodic or wall; a wall mirrors aux without a sign
+   * change), applied to every aux plane. */
   int32_t num_aux;
   int32_t fwave;
   int32_t aux_bc_lower[2];
   int32_t aux_bc_upper[2];
   double dry_tolerance;     /* problem_data['dry_tolerance']: h <= dry_tolerance has zero velocity */
+  /* method(6) = mcapa: the 1-based aux plane holding the capacity function kappa (pyclaw's
+   * state.index_capa + 1), 0 = none.  With a capacity function every Riemann solver takes aux
+   * arrays (num_aux >= mcapa), the step is the capacity form of step2/step2ds (each cell's update
+   * divided by its kappa, dt/dx per cell in qadd, the CFL number and the transverse terms, the
+   * average of the two cells' dt/(dx kappa) in the second-order correction), and kappa must be
+   * positive in every cell and ghost cell.  Accepted by the _aux entry points; the padded, batched
+   * and adjoint entry points reject it. */
+  int32_t mcapa;
+  int32_t reserved_capa;
 } claw2d_desc;
 
 /* Device-resident time-loop control block (one per grid). */

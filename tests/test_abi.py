@@ -182,3 +182,30 @@ def test_sweep_padded_checks_arguments():
     assert lib.claw2d_sweep_padded(C.byref(bad), qa, qb, None, 0.1, 1, ctrl, None) == -10
     aux_d = _desc(solver_id=2, num_aux=1, fwave=1, bc_lower=(0, 3), aux_bc_lower=(1, 3), aux_bc_upper=(1, 1))
     assert lib.claw2d_sweep_padded(C.byref(aux_d), qa, qb, None, 0.1, 2, ctrl, None) == -22
+
+
+def test_capacity_descriptor_rules():
+    """mcapa (method(6), 1-based aux plane of kappa): any solver takes aux arrays with a capacity
+    function, mcapa must name an aux plane, and the padded, batched and adjoint entry points refuse
+    it before any device call."""
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    ok = _desc(num_aux=1, index_capa=0)
+    assert ok.mcapa == 1 and lib.claw2d_validate(C.byref(ok)) == 0
+    assert lib.claw2d_validate(C.byref(_desc(solver_id=1, num_eqn=4, num_waves=4, num_aux=2,
+                                             index_capa=1))) == 0
+    assert lib.claw2d_validate(C.byref(_bathy(num_aux=2, index_capa=1))) == 0
+    bad = _desc(num_aux=1)
+    bad.mcapa = 2
+    assert lib.claw2d_validate(C.byref(bad)) == -29
+    bad.mcapa = -1
+    assert lib.claw2d_validate(C.byref(bad)) == -29
+    assert lib.claw2d_validate(C.byref(_desc(num_aux=1))) == -16  # aux without kappa: no reader
+    assert lib.claw2d_validate(C.byref(_desc(num_aux=1, index_capa=0, aux_bc_lower=(0, 1)))) == -19
+    qa, qb, ctrl, aux = C.c_void_p(4096), C.c_void_p(8192), C.c_void_p(12288), C.c_void_p(16384)
+    assert lib.claw2d_step_padded(C.byref(ok), qa, qb, aux, 0.1, ctrl, None) == -28
+    assert lib.claw2d_sweep_padded(C.byref(ok), qa, qb, aux, 0.1, 1, ctrl, None) == -28
+    assert lib.claw2d_step(C.byref(ok), qa, qb, 0.1, ctrl, None) == -26  # needs the _aux form
+    assert lib.claw2d_adjoint_workspace_bytes(C.byref(ok)) == 0
+    assert lib.claw2d_batch_advance(C.byref(ok), qa, 1, 1, qa, qb, ctrl, aux, 1, None) == -25
+

@@ -132,7 +132,9 @@ def test_bathymetry_plugin_metadata():
     (lambda s, st: s.aux_bc_upper.__setitem__(1, pyclaw.BC.custom), NotImplementedError, "custom"),
     (lambda s, st: s.aux_bc_lower.__setitem__(1, pyclaw.BC.periodic), Exception, "Periodic aux"),
     (lambda s, st: st.problem_data.pop("dry_tolerance"), KeyError, "dry_tolerance"),
-    (lambda s, st: setattr(st, "index_capa", 0), NotImplementedError, "capacity"),
+    (lambda s, st: setattr(st, "index_capa", 1), ValueError, "index_capa"),
+    (lambda s, st: (setattr(st, "index_capa", 0), s.bc_lower.__setitem__(0, pyclaw.BC.custom),
+                    setattr(s, "user_bc_lower", lambda *a: None)), NotImplementedError, "capacity"),
 ])
 def test_bathymetry_configuration_checks(change, exc, match):
     solver, state, domain, sol = make_bathy()
