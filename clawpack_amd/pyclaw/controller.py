@@ -1,6 +1,7 @@
 """Controller (upstream pyclaw/src/pyclaw/controller.py): output-time loop around
 solver.evolve_to_time.  Frames are written to outdir in the ascii format (output_format='ascii',
-the default; None writes nothing) and/or kept in memory (keep_copy)."""
+the default), the binary format ('binary'), or both (a list); None writes nothing.  Frames can
+also be kept in memory (keep_copy)."""
 import copy
 
 import numpy as np

@@ -41,19 +41,40 @@ def write(solution, frame, path, file_prefix="fort", write_aux=False, options=No
         _write_array(os.path.join(path, aname), solution, lambda s: s.aux)
 
 
+def _write_header(f, patch):
+    """The patch header of a frame file (shared with the binary format)."""
+    f.write("%5i                  patch_number\n" % patch.patch_index)
+    f.write("%5i                  AMR_level\n" % patch.level)
+    for d in patch.dimensions:
+        f.write("%5i                  m%s\n" % (d.num_cells, d.name))
+    for d in patch.dimensions:
+        f.write("%18.8e     %slow\n" % (d.lower, d.name))
+    for d in patch.dimensions:
+        f.write("%18.8e     d%s\n" % (d.delta, d.name))
+    f.write("\n")
+
+
+def _read_header(lines, k, num_dim):
+    """Parses the patch header starting at or after line k; returns (header dict, next line)."""
+    while not lines[k].strip():
+        k += 1
+    hdr = {"patch_number": int(lines[k].split()[0]), "AMR_level": int(lines[k + 1].split()[0])}
+    k += 2
+    n = [int(lines[k + d].split()[0]) for d in range(num_dim)]
+    k += num_dim
+    lower = [float(lines[k + d].split()[0]) for d in range(num_dim)]
+    k += num_dim
+    delta = [float(lines[k + d].split()[0]) for d in range(num_dim)]
+    k += num_dim
+    hdr.update(num_cells=n, lower=lower, delta=delta)
+    return hdr, k
+
+
 def _write_array(fname, solution, getter):
     with open(fname, "w") as f:
         for state in solution.states:
             patch = state.patch
-            f.write("%5i                  patch_number\n" % patch.patch_index)
-            f.write("%5i                  AMR_level\n" % patch.level)
-            for d in patch.dimensions:
-                f.write("%5i                  m%s\n" % (d.num_cells, d.name))
-            for d in patch.dimensions:
-                f.write("%18.8e     %slow\n" % (d.lower, d.name))
-            for d in patch.dimensions:
-                f.write("%18.8e     d%s\n" % (d.delta, d.name))
-            f.write("\n")
+            _write_header(f, patch)
             a = np.asarray(getter(state))
             nv = a.shape[0]
             if patch.num_dim == 1:
@@ -100,17 +121,8 @@ def _read_array(fname, nv, nstates, num_dim):
     k = 0
     out = []
     for _ in range(nstates):
-        while not lines[k].strip():
-            k += 1
-        hdr = {"patch_number": int(lines[k].split()[0]), "AMR_level": int(lines[k + 1].split()[0])}
-        k += 2
-        n = [int(lines[k + d].split()[0]) for d in range(num_dim)]
-        k += num_dim
-        lower = [float(lines[k + d].split()[0]) for d in range(num_dim)]
-        k += num_dim
-        delta = [float(lines[k + d].split()[0]) for d in range(num_dim)]
-        k += num_dim
-        hdr.update(num_cells=n, lower=lower, delta=delta)
+        hdr, k = _read_header(lines, k, num_dim)
+        n = hdr["num_cells"]
         ncell = int(np.prod(n))
         vals = []
         while len(vals) < ncell * nv:

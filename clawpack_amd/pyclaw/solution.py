@@ -3,15 +3,24 @@ import copy
 
 import numpy as np
 
-from .fileio import ascii
+from .fileio import ascii, binary
 from .geometry import Dimension, Domain, Patch
 from .state import State
+
+
+_FORMATS = {"ascii": ascii, "binary": binary}
+
+
+def _format(name):
+    if name not in _FORMATS:
+        raise NotImplementedError("file_format %r (the 'ascii' and 'binary' frame formats are supported)" % name)
+    return _FORMATS[name]
 
 
 class Solution:
     """Solution(state, domain), Solution(state) (the state's patch as domain), or
     Solution(frame, path='./_output', file_format='ascii', file_prefix=None, read_aux=True) to
-    read a frame written by Solution.write / Controller."""
+    read a frame written by Solution.write / Controller (file_format 'ascii' or 'binary')."""
 
     def __init__(self, *args, **kwargs):
         if len(args) >= 1 and isinstance(args[0], (int, np.integer)):
@@ -26,9 +35,7 @@ class Solution:
 
     def _read(self, frame, path="./_output", file_format="ascii", file_prefix=None, read_aux=True,
               options=None):
-        if file_format != "ascii":
-            raise NotImplementedError("file_format %r (only 'ascii' frames are supported)" % file_format)
-        t, num_eqn, num_aux, num_dim, patches, aux = ascii.read(
+        t, num_eqn, num_aux, num_dim, patches, aux = _format(file_format).read(
             None, frame, path, file_prefix or "fort", read_aux, options)
         self.states = []
         for k, (hdr, q) in enumerate(patches):
@@ -47,12 +54,12 @@ class Solution:
 
     def write(self, frame, path="./_output", file_format="ascii", file_prefix=None, write_aux=False,
               options=None, write_p=False):
-        """Write frame number `frame` (upstream Solution.write); ascii format only."""
+        """Write frame number `frame` (upstream Solution.write) in each of the formats named by
+        file_format ('ascii', 'binary', or a list of them)."""
         formats = [file_format] if isinstance(file_format, str) else list(file_format)
-        for fmt in formats:
-            if fmt != "ascii":
-                raise NotImplementedError("file_format %r (only 'ascii' frames are supported)" % fmt)
-            ascii.write(self, frame, path, file_prefix or "fort", write_aux, options, write_p)
+        writers = [_format(fmt) for fmt in formats]  # refuse an unknown format before writing any
+        for w in writers:
+            w.write(self, frame, path, file_prefix or "fort", write_aux, options, write_p)
 
     @property
     def state(self):
