@@ -43,6 +43,11 @@ def parse():
     p.add_argument("--prewarm", type=float, default=0.5,
                    help="seconds of untimed steps of each kernel before its counted warm-up")
     p.add_argument("--n", type=int, default=4096, help="cells per side")
+    p.add_argument("--no-tune", action="store_true",
+                   help="skip the tile-shape autotuner (profiling runs: its candidates would share "
+                        "the step kernel's name in a trace); the shape is --shape or the default")
+    p.add_argument("--shape", default=None, metavar="SEG_H,WG_WAVES,ORDER",
+                   help="tile shape of the headline FP64 kernel when not tuned")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
@@ -132,17 +137,29 @@ def prewarm(g, q, dt0, seconds, aux=None):
     return el
 
 
-def run_gpu(n, dtype_id, steps, warmup, seed, split=False, prewarm_s=0.0):
+def run_gpu(n, dtype_id, steps, warmup, seed, split=False, prewarm_s=0.0, tune=True, shape=None):
+    """Times `steps` evolve_to_time attempts of the headline workload.  tune: the tile shape
+    (rows per wavefront, wavefronts per workgroup, launch order) is chosen by the autotuner,
+    measured here on this workload (the library default is one of the candidates)."""
     import torch
-    from clawpack_amd import problems
+    from clawpack_amd import autotune, problems
     from clawpack_amd.device import DeviceGrid, make_desc
 
     q, dx, dy = problems.gaussian_humps(n, n, seed)
+    # the FP32 state stores the depth as h - 1 (the humps' still-water level), as the precision
+    # leg measures it
     d = make_desc(0, dtype_id, n, n, dx, dy, 3, 3, order=2, transverse_waves=2, mthlim=(4, 4, 4),
-                  bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
+                  bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0, depth_ref=1.0 if dtype_id == 1 else 0.0)
     g = DeviceGrid(d, "cuda:0", split=split)
     g.upload(q)
     warmed = prewarm(g, q, 1e-4, prewarm_s) if prewarm_s > 0 else 0.0
+    shape_ms = None
+    if tune:  # measured after the pre-warm, with the clocks settled on this kernel
+        shape, shape_ms = autotune.tune_grid(d, q, 1e-4, steps=30, rounds=3, use_cache=False, split=split)
+        torch.cuda.empty_cache()
+    if shape is None:
+        shape = {"seg_h": 0, "wg_waves": 4, "launch_order": 0}  # the library's default shape
+    g.desc = autotune.apply_shape(type(d).from_buffer_copy(bytes(d)), shape)
     del q
     # dt from the initial wave speed, as after pyclaw's first accepted step
     g.init_ctrl(0.0, 1e-4, target_steps=warmup + steps + 10**9)
@@ -164,7 +181,7 @@ def run_gpu(n, dtype_id, steps, warmup, seed, split=False, prewarm_s=0.0):
     attempts = int(c1.attempts - c0.attempts)
     assert c1.status == 0, "device status %d" % c1.status
     return dict(ms=ms, wall=wall, accepted=accepted, attempts=attempts, cells=n * n,
-                t=c1.t, dt=c1.dt, cfl=c1.cfl, prewarm_s=warmed)
+                t=c1.t, dt=c1.dt, cfl=c1.cfl, prewarm_s=warmed, tile_shape=shape, tile_shape_ms=shape_ms)
 
 
 def run_batch(ngrids, dtype_id, steps, seq_steps, seed):
@@ -231,6 +248,7 @@ def fp32_vs_fp64(make, q, dt, steps, comps, aux=None, split=False):
         c = g.read_ctrl()
         assert c.status == 0 and c.cfl <= 1.0, "status %d cfl %g" % (c.status, c.cfl)
         planes[dtype_id] = g.planes().to(torch.float64).clone()
+        planes[dtype_id][0] += g.desc.depth_ref  # a stored depth deviation back to h
         del g
     a, b = planes[0], planes[1]
     rec = {}
@@ -359,9 +377,12 @@ def run_precision(sizes, steps, seed):
         q, dx, dy = problems.gaussian_humps(n, n, seed)
         dt = 0.8 * min(dx, dy) / (math.sqrt(float(q[0].max())) + 0.5)
         grids, rec = {}, {}
-        for dtype_id, name in ((0, "fp64"), (1, "fp32")):
+        # fp32: the FP32 path as offered, depth stored as h - 1 (the still-water level of the
+        # humps); fp32_stored_h: the same kernels storing h itself, for comparison
+        for dtype_id, name, ref in ((0, "fp64", 0.0), (1, "fp32", 1.0), (1, "fp32_stored_h", 0.0)):
             d = make_desc(0, dtype_id, n, n, dx, dy, 3, 3, order=2, transverse_waves=2,
-                          mthlim=(4, 4, 4), bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0, dt_variable=False)
+                          mthlim=(4, 4, 4), bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0, dt_variable=False,
+                          depth_ref=ref)
             g = DeviceGrid(d, "cuda:0")
             g.upload(q)
             g.init_ctrl(0.0, dt, target_steps=steps)
@@ -374,27 +395,30 @@ def run_precision(sizes, steps, seed):
             c = g.read_ctrl()
             assert c.status == 0 and c.numsteps == steps, "status %d steps %d" % (c.status, c.numsteps)
             grids[name] = g
-            rec[name] = {"mass_rel_drift_max": drift}
+            rec[name] = {"mass_rel_drift_max": drift, "depth_ref": ref}
         a = grids["fp64"].planes()
-        b = grids["fp32"].planes().to(torch.float64)
-        err = {}
-        qmax = float(a.abs().max())
-        dall = (b - a).abs()
-        for m, comp in enumerate(("h", "hu", "hv")):
-            diff = dall[m]
-            err[comp] = {"L1_rel": float(diff.sum() / a[m].abs().sum()),
-                         "Linf_rel": float(diff.max() / a[m].abs().max())}
-        # the state as a whole: max |dq| / max |q| and sum |dq| / sum |q| over all components
-        err["q"] = {"Linf_rel_to_max_q": float(dall.max()) / qmax,
-                    "L1_rel": float(dall.sum() / a.abs().sum())}
-        rec["fp32_vs_fp64"] = {
-            "max_component_Linf_rel": max(err[c]["Linf_rel"] for c in ("h", "hu", "hv")),
-            "max_component_L1_rel": max(err[c]["L1_rel"] for c in ("h", "hu", "hv")),
-            "per_component": {c: err[c] for c in ("h", "hu", "hv")}, "whole_state": err["q"]}
+        for name, key in (("fp32", "fp32_vs_fp64"), ("fp32_stored_h", "fp32_stored_h_vs_fp64")):
+            b = grids[name].planes().to(torch.float64)
+            b[0] += grids[name].desc.depth_ref  # stored h - depth_ref -> h
+            err = {}
+            qmax = float(a.abs().max())
+            dall = (b - a).abs()
+            for m, comp in enumerate(("h", "hu", "hv")):
+                diff = dall[m]
+                err[comp] = {"L1_rel": float(diff.sum() / a[m].abs().sum()),
+                             "Linf_rel": float(diff.max() / a[m].abs().max())}
+            # the state as a whole: max |dq| / max |q| and sum |dq| / sum |q| over all components
+            err["q"] = {"Linf_rel_to_max_q": float(dall.max()) / qmax,
+                        "L1_rel": float(dall.sum() / a.abs().sum())}
+            rec[key] = {
+                "max_component_Linf_rel": max(err[c]["Linf_rel"] for c in ("h", "hu", "hv")),
+                "max_component_L1_rel": max(err[c]["L1_rel"] for c in ("h", "hu", "hv")),
+                "per_component": {c: err[c] for c in ("h", "hu", "hv")}, "whole_state": err["q"]}
+            del b
         rec["steps"] = steps
         rec["dt"] = dt
         out["%dx%d" % (n, n)] = rec
-        del grids, a, b
+        del grids, a
         torch.cuda.empty_cache()
     return out
 
@@ -466,6 +490,22 @@ def run_adjoint(n, nsteps, seed, fd_dirs):
                       "median_rel_err_vs_directional": float(np.median(rel_dir)) if rel_dir else None,
                       "max_rel_err_vs_directional": max(rel_dir) if rel_dir else None,
                       "max_abs_err_over_grad_norm": max(rel_norm) if rel_norm else None}}
+        if name == "unlimited" and dirs:
+            # Richardson-extrapolated central differences, (4 D(e/2) - D(e)) / 3 with e = 1e-3:
+            # truncation O(e^4) and a rounding error of about |L| 1e-16 / e, both far below those
+            # of the plain e = 1e-6 difference on this smooth loss
+            er, rel_r = 1e-3, []
+            for v in dirs:
+                with torch.no_grad():
+                    def D(e):
+                        return (float(loss_of(torch.tensor(eta0 + e * v, device="cuda")))
+                                - float(loss_of(torch.tensor(eta0 - e * v, device="cuda")))) / (2 * e)
+                    fd = (4.0 * D(er / 2) - D(er)) / 3.0
+                ad = float((grad * v).sum())
+                rel_r.append(abs(fd - ad) / abs(ad))
+            rec["fd_richardson"] = {"eps": er, "directions": len(dirs),
+                                    "median_rel_err_vs_directional": float(np.median(rel_r)),
+                                    "max_rel_err_vs_directional": max(rel_r)}
         if fwd:
             f, b = float(np.median(fwd)), float(np.median(bwd))
             rec.update(forward_ms=f * 1e3, backward_ms=b * 1e3, backward_over_forward=b / f,
@@ -516,7 +556,12 @@ def main():
     if a.copy_probe:
         copy_probe(a.copy_probe)
         return
-    r64 = run_gpu(a.n, 0, a.steps, a.warmup, a.seed, prewarm_s=a.prewarm)
+    forced = None
+    if a.shape:
+        h, w, o = (int(x) for x in a.shape.split(","))
+        forced = {"seg_h": h, "wg_waves": w, "launch_order": o}
+    r64 = run_gpu(a.n, 0, a.steps, a.warmup, a.seed, prewarm_s=a.prewarm, tune=not a.no_tune,
+                  shape=forced)
     progress("fp64 step: %.4f ms/attempt" % (r64["ms"] / r64["attempts"]))
     ms_step = r64["ms"] / r64["accepted"]
     value = r64["cells"] * r64["accepted"] / (r64["ms"] * 1e-3)
@@ -539,7 +584,9 @@ def main():
         "data": "synthetic (seeded Gaussian humps)",
         "config": {"workload": "swe_roe_efix_unsplit_mc_periodic_humps16_%dx%d_fp64" % (a.n, a.n),
                    "cells": r64["cells"], "accepted_steps": r64["accepted"],
-                   "attempts": r64["attempts"], "parallelism": "single GPU"},
+                   "attempts": r64["attempts"], "parallelism": "single GPU",
+                   "tile_shape": r64["tile_shape"]},
+        "autotune": {"fp64": r64["tile_shape_ms"]} if r64["tile_shape_ms"] else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(a.n),
                      "kernel": "step2_fused<double,SweP<double>,4,false,2> (one launch per attempt)",
@@ -549,18 +596,22 @@ def main():
     out["roofline"]["measured_copy_gbs"] = copy_gbs
     out["roofline"]["frac_of_measured_copy"] = achieved / copy_gbs
     if not a.no_fp32:
-        r32 = run_gpu(a.n, 1, a.steps, a.warmup, a.seed, prewarm_s=a.prewarm)
+        r32 = run_gpu(a.n, 1, a.steps, a.warmup, a.seed, prewarm_s=a.prewarm, tune=not a.no_tune)
         v32 = r32["cells"] * r32["accepted"] / (r32["ms"] * 1e-3)
         progress("fp32 step: %.4f ms/attempt" % (r32["ms"] / r32["attempts"]))
         out["fp32"] = {"value": v32, "ms_per_step": r32["ms"] / r32["accepted"],
                        "ratio_vs_fp64": v32 / value,
-                       "roofline_frac": 24.0 * v32 / 1e9 / HBM_PEAK_GBS}
+                       "roofline_frac": 24.0 * v32 / 1e9 / HBM_PEAK_GBS,
+                       "tile_shape": r32["tile_shape"], "depth_ref": 1.0}
+        if out["autotune"] is not None:
+            out["autotune"]["fp32"] = r32["tile_shape_ms"]
     if not a.no_split:
         sp = {}
         for dt_id, name, bpc in ((0, "fp64", 48.0), (1, "fp32", 24.0)):
-            r = run_gpu(a.n, dt_id, a.steps, a.warmup, a.seed, split=True, prewarm_s=a.prewarm)
+            r = run_gpu(a.n, dt_id, a.steps, a.warmup, a.seed, split=True, prewarm_s=a.prewarm,
+                        tune=not a.no_tune)
             sp[name] = {"cell_updates_per_s": r["cells"] * r["accepted"] / (r["ms"] * 1e-3),
-                        "ms_per_step": r["ms"] / r["accepted"],
+                        "ms_per_step": r["ms"] / r["accepted"], "tile_shape": r["tile_shape"],
                         "roofline_frac": bpc * r["cells"] * r["attempts"] / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
         sp["workload"] = "same as the headline line with dimensional_split=True (step2ds, one pass per step)"
         out["split"] = sp

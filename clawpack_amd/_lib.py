@@ -26,7 +26,8 @@ class ClawDesc(C.Structure):
         ("cfl_max", C.c_double), ("cfl_desired", C.c_double), ("dt_max", C.c_double),
         ("num_aux", C.c_int32), ("fwave", C.c_int32), ("aux_bc_lower", C.c_int32 * 2),
         ("aux_bc_upper", C.c_int32 * 2), ("dry_tolerance", C.c_double),
-        ("mcapa", C.c_int32), ("reserved_capa", C.c_int32),
+        ("mcapa", C.c_int32), ("reserved_capa", C.c_int32), ("depth_ref", C.c_double),
+        ("wg_waves", C.c_int32), ("launch_order", C.c_int32),
     ]
 
 

@@ -1,11 +1,19 @@
-"""Autotuner for the marching-strip decomposition (rows per work item, `seg_h`).
+"""Autotuner for the step kernels' tile shape and schedule.
 
-The step kernel's cost per wavefront is proportional to its rows; seg_h trades halo rows
-(4 per segment) and per-segment warm-up against the number of wavefronts in flight and the
-balance of the last round.  The library's default picks seg_h from the kernel's occupancy so
-every wavefront is resident at once; this module measures candidates on the device with HIP
-events and caches the winner in JSON keyed by device name, HIP runtime version, solver,
-precision and size class (sides rounded up to powers of two).
+A single-grid launch is a set of wavefronts, each marching a strip of 60 columns over a segment of
+`seg_h` rows; the shape has three knobs, none of which changes a result bit:
+
+* seg_h, rows per wavefront: trades halo rows (4 per segment) and per-segment warm-up against
+  the number of wavefronts in flight and the balance of the last round (0 = the library's
+  occupancy-derived default);
+* wg_waves, wavefronts per workgroup (1, 2 or 4): how the wavefronts are packed onto CUs;
+* launch_order: XCD-contiguous strips (0, neighbouring strips share halo lines in one L2) or
+  the plain dispatch order (1).
+
+tune_grid measures seg_h candidates at the default packing, then the packings and orders at the
+best height, on the device with HIP events, and caches the winner in JSON keyed by device name,
+HIP runtime version, solver, precision, size class (sides rounded up to powers of two) and the
+method.  tune_batch does the same for the rows per work item of the batched kernel.
 
 Cache file: $CLAWPACK_AMD_TUNE_CACHE or ~/.cache/clawpack_amd/autotune.json.
 """
@@ -65,44 +73,87 @@ def key(desc, kind="grid", split=False):
                      "aux%d" % desc.num_aux])
 
 
-def time_seg_h(desc, q, seg_h, dt, steps=10, warmup=2, split=False):
-    """ms per fixed-dt step of the single-grid kernel (split: the step2ds kernel) with
-    desc.seg_h = seg_h."""
+SHAPES = ((4, 0), (2, 0), (1, 0), (4, 1), (2, 1))  # (wg_waves, launch_order)
+
+
+def _with_shape(desc, seg_h, wg_waves=0, launch_order=0):
     d = type(desc).from_buffer_copy(bytes(desc))
-    d.seg_h = int(seg_h)
-    g = DeviceGrid(d, split=split)
-    g.upload(q)
-    g.init_ctrl(0.0, dt)
-    for _ in range(warmup):
-        g.enqueue_step(dt)
-    s = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(steps):
-        g.enqueue_step(dt)
-    e1.record(s)
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps
+    d.seg_h, d.wg_waves, d.launch_order = int(seg_h), int(wg_waves), int(launch_order)
+    return d
 
 
-def tune_grid(desc, q, dt, candidates=DEFAULT_CANDIDATES, steps=10, use_cache=True, split=False):
-    """Best seg_h for this descriptor (0 = the occupancy-derived default) and step form.  Returns
-    (seg_h, table of ms per step by candidate)."""
+def time_shape(desc, q, dt, seg_h, wg_waves=0, launch_order=0, steps=10, warmup=2, split=False, grid=None):
+    """ms per fixed-dt step of the single-grid kernel (split: the step2ds kernel) in this shape.
+    grid: a DeviceGrid of this descriptor's size to run on (its state is stepped, not restored;
+    the shape never changes a result, and only the time is kept); None makes one from q."""
+    g = grid
+    if g is None:
+        g = DeviceGrid(desc, split=split)
+        g.upload(q)
+    saved = g.desc
+    g.desc = _with_shape(desc, seg_h, wg_waves, launch_order)
+    try:
+        g.init_ctrl(0.0, dt)
+        for _ in range(warmup):
+            g.enqueue_step(dt)
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(steps):
+            g.enqueue_step(dt)
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+    finally:
+        g.desc = saved
+
+
+def time_seg_h(desc, q, seg_h, dt, steps=10, warmup=2, split=False):
+    """ms per fixed-dt step with desc.seg_h = seg_h (default packing and order)."""
+    return time_shape(desc, q, dt, seg_h, steps=steps, warmup=warmup, split=split)
+
+
+def shape_name(seg_h, wg_waves, launch_order):
+    return "h%d/w%d/%s" % (int(seg_h), int(wg_waves) or 4, "xcd" if int(launch_order) == 0 else "plain")
+
+
+def tune_grid(desc, q, dt, candidates=DEFAULT_CANDIDATES, steps=10, use_cache=True, split=False,
+              shapes=SHAPES, rounds=2):
+    """Best tile shape for this descriptor and step form: seg_h over `candidates` (0 = the
+    occupancy-derived default) at 4 wavefronts per workgroup in XCD order, then the (wg_waves,
+    launch_order) pairs of `shapes` at that height.  Every candidate runs on one grid (one upload)
+    `rounds` times, interleaved with the others, and keeps its fastest round.  Returns (shape dict
+    with keys seg_h, wg_waves, launch_order, and the ms per step of every measured shape by
+    shape_name)."""
     k = key(desc, split=split)
     table = _load() if use_cache else {}
-    if use_cache and k in table:
-        return int(table[k]["seg_h"]), table[k]["ms"]
+    if use_cache and k in table and "wg_waves" in table[k]:
+        e = table[k]
+        return {"seg_h": int(e["seg_h"]), "wg_waves": int(e["wg_waves"]),
+                "launch_order": int(e["launch_order"])}, e["ms"]
+    g = DeviceGrid(desc, split=split)
+    g.upload(q)
     ms = {}
-    for c in candidates:
-        if c and c > desc.my:
-            continue
-        ms[str(c)] = time_seg_h(desc, q, c, dt, steps=steps, split=split)
-    best = int(min(ms, key=ms.get))
+
+    def measure(cands):
+        for _ in range(rounds):
+            for c in cands:
+                t = time_shape(desc, q, dt, *c, steps=steps, split=split, grid=g)
+                name = shape_name(*c)
+                ms[name] = (min(t, ms[name][0]) if name in ms else t, c)
+
+    measure([(c, 4, 0) for c in candidates if not (c and c > desc.my)])
+    best = min(ms.values())[1]
+    measure([(best[0], wpb, order) for wpb, order in shapes if shape_name(best[0], wpb, order) not in ms])
+    seg_h, wpb, order = min(ms.values())[1]
+    del g
+    shape = {"seg_h": seg_h, "wg_waves": wpb, "launch_order": order}
+    times = {n: v[0] for n, v in ms.items()}
     if use_cache:
         table = _load()
-        table[k] = {"seg_h": best, "ms": ms}
+        table[k] = dict(shape, ms=times)
         _store(table)
-    return best, ms
+    return shape, times
 
 
 def tune_batch(desc, grids, candidates=(16, 32, 48, 64, 128), attempts=5, use_cache=True):
@@ -134,13 +185,14 @@ def tune_batch(desc, grids, candidates=(16, 32, 48, 64, 128), attempts=5, use_ca
     return best, ms
 
 
-def tuned_seg_h(desc, q=None, dt=None, split=False):
-    """Cached seg_h for desc and step form (tuning on a synthetic state of the right shape if
-    needed)."""
+def tuned_shape(desc, q=None, dt=None, split=False):
+    """Cached tile shape for desc and step form (tuning on a synthetic state of the right shape if
+    needed): dict with seg_h, wg_waves and launch_order."""
     k = key(desc, split=split)
     table = _load()
-    if k in table:
-        return int(table[k]["seg_h"])
+    if k in table and "wg_waves" in table[k]:
+        e = table[k]
+        return {"seg_h": int(e["seg_h"]), "wg_waves": int(e["wg_waves"]), "launch_order": int(e["launch_order"])}
     if q is None:
         neq = desc.num_eqn
         q = np.ones((neq, desc.mx, desc.my), order="F")
@@ -149,3 +201,15 @@ def tuned_seg_h(desc, q=None, dt=None, split=False):
     if dt is None:
         dt = 0.1 * min(desc.dx, desc.dy)
     return tune_grid(desc, q, dt, split=split)[0]
+
+
+def tuned_seg_h(desc, q=None, dt=None, split=False):
+    """The rows per wavefront of tuned_shape."""
+    return tuned_shape(desc, q, dt, split=split)["seg_h"]
+
+
+def apply_shape(desc, shape):
+    """Sets the tuned shape's fields on desc (in place) and returns it."""
+    desc.seg_h, desc.wg_waves, desc.launch_order = (int(shape["seg_h"]), int(shape["wg_waves"]),
+                                                    int(shape["launch_order"]))
+    return desc

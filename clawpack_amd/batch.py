@@ -58,7 +58,10 @@ class BatchedGrids:
         host = torch.empty(off, dtype=self.torch_dtype)
         for g, (q, dx, dy) in enumerate(grids):
             mx, my = self.shapes[g]
-            planes = torch.from_numpy(np.ascontiguousarray(np.asarray(q, dtype=np.float64))).permute(0, 2, 1)
+            qq = torch.from_numpy(np.array(q, dtype=np.float64))
+            if desc.depth_ref != 0.0:
+                qq[0] -= desc.depth_ref  # stored depth h - depth_ref (FP32 shallow water)
+            planes = qq.permute(0, 2, 1)
             host[self.offsets[g]:self.offsets[g] + self.neq * mx * my] = \
                 planes.contiguous().reshape(-1).to(self.torch_dtype)
         self.q[0].copy_(host)
@@ -135,6 +138,8 @@ class BatchedGrids:
         buf = self.q[int(c.cur)]
         flat = buf[self.offsets[g]:self.offsets[g] + self.neq * mx * my]
         t = flat.reshape(self.neq, my, mx).to(torch.float64).cpu()
+        if self.desc.depth_ref != 0.0:
+            t[0] += self.desc.depth_ref
         return np.asfortranarray(t.permute(0, 2, 1).numpy())
 
     def cells(self):

@@ -3,15 +3,16 @@
 // lam_out = (d q^{n+1} / d q^n)^T lam_in, with q^{n+1} the step of claw2d_step from q^n.  The
 // step is a sum of per-interface contributions (see adjoint.hpp), so its transpose is a sum of
 // per-interface reverses, scattered onto the four cells each interface reads.  Kernels:
-//   adj_sweep<0>  one wavefront per (row line, segment of 58 columns).  Lane = column = the
-//                 x-interface on its left.  Each lane computes its interface's Roe split once; the
-//                 limiter partners' data and, in reverse, their adjoints travel one lane left or
-//                 right over ds_bpermute; the cell adjoint of column c is the sum of the shares of
-//                 interfaces c and c+1.  Writes the x-sweep share of every extended cell
-//                 (interior + 2 ghost layers) to `ext`, a (my+4) x (mx+4) workspace.
-//   adj_sweep<1>  the same along columns (lane = row position, strided loads; the waves of a
-//                 workgroup take adjacent columns), adding the y-sweep share into `ext`.
-//   adj_fold      lam_out = lam_in + ext(interior) + the ghost-cell shares of ext folded back to
+//   adj_sweeps    both sweep directions in one launch (they are independent).  x part: one
+//                 wavefront per (row line, segment of 58 columns); lane = column = the x-interface
+//                 on its left.  Each lane computes its interface's Roe split once; the limiter
+//                 partners' data and, in reverse, their adjoints travel one lane left or right over
+//                 ds_bpermute; the cell adjoint of column c is the sum of the shares of interfaces
+//                 c and c+1.  It writes the x-sweep share of every extended cell (interior + 2
+//                 ghost layers) to the workspace's first three (my+4) x (mx+4) planes.  y part: the
+//                 same along columns (lane = row position, strided loads; the waves of a
+//                 workgroup take adjacent columns), into the next three planes.
+//   adj_fold      lam_out = lam_in + both shares (interior) + the ghost-cell shares folded back to
 //                 the interior cells they were copied from (extrap / periodic / wall, with the
 //                 wall's sign), in a fixed order (deterministic, no atomics).
 // Branches (entropy fix, limiter cases, upwind side, transverse speed signs) are taken as the
@@ -92,8 +93,10 @@ __device__ __forceinline__ void load_lam(const AdjArgs& a, int x, int y, double 
   o[2] = DIR == 0 ? c2 : c1;
 }
 
+// One sweep direction's share of the reverse step; block `blk` of the `nblk` this direction has.
+// Direction DIR writes its own workspace planes (ext + 3 DIR eplane); adj_fold adds the two.
 template <int DIR>
-__global__ void __launch_bounds__(64 * kWPBA) adj_sweep(AdjArgs a) {
+__device__ __forceinline__ void sweep_body(const AdjArgs& a, const int blk, const int nblk) {
   const int lane = threadIdx.x & 63;
   const int n_al = DIR == 0 ? a.mx : a.my;  // cells along the sweep line
   const int n_ln = DIR == 0 ? a.my : a.mx;  // sweep lines
@@ -102,11 +105,11 @@ __global__ void __launch_bounds__(64 * kWPBA) adj_sweep(AdjArgs a) {
   // access is strided by the row pitch): the waves of a workgroup take adjacent columns of one
   // segment, and the workgroups that land on one XCD (dealt to the 8 XCDs in turn) take
   // consecutive column groups, so the cache lines of neighbouring columns are shared in that
-  // XCD's L2 instead of being fetched by every XCD.  `bb` is a bijection of [0, gridDim.x).
+  // XCD's L2 instead of being fetched by every XCD.  `bb` is a bijection of [0, nblk).
   // (The same renumbering for DIR 0 measured 9% slower: its accesses are already coalesced.)
-  int bb = blockIdx.x;
+  int bb = blk;
   if (DIR == 1) {
-    const int nb = gridDim.x, q8 = nb >> 3, r8 = nb & 7, x = blockIdx.x & 7, i = blockIdx.x >> 3;
+    const int nb = nblk, q8 = nb >> 3, r8 = nb & 7, x = blk & 7, i = blk >> 3;
     bb = x < r8 ? x * (q8 + 1) + i : r8 * (q8 + 1) + (x - r8) * q8 + i;
   }
   const int wave = __builtin_amdgcn_readfirstlane(bb * kWPBA + (threadIdx.x >> 6));
@@ -119,11 +122,12 @@ __global__ void __launch_bounds__(64 * kWPBA) adj_sweep(AdjArgs a) {
   const int ew = a.mx + 4;
   const size_t eplane = (size_t)ew * (size_t)(a.my + 4);
   const size_t eoff = DIR == 0 ? (size_t)(line + 2) * ew + (size_t)(p + 2) : (size_t)(p + 2) * ew + (size_t)(line + 2);
+  double* const ext = a.ext + (size_t)(3 * DIR) * eplane;
   // the forward sweeps lines -1 .. n_ln only
   if (line < -1 || line > n_ln) {
-    if (DIR == 0 && own) {
+    if (own) {
 #pragma unroll
-      for (int m = 0; m < 3; ++m) a.ext[(size_t)m * eplane + eoff] = 0.0;
+      for (int m = 0; m < 3; ++m) ext[(size_t)m * eplane + eoff] = 0.0;
     }
     return;
   }
@@ -190,11 +194,17 @@ __global__ void __launch_bounds__(64 * kWPBA) adj_sweep(AdjArgs a) {
   if (own) {
     const double c[3] = {o[0], DIR == 0 ? o[1] : o[2], DIR == 0 ? o[2] : o[1]};
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      double* e = a.ext + (size_t)m * eplane + eoff;
-      if (DIR == 0) *e = c[m]; else *e += c[m];
-    }
+    for (int m = 0; m < 3; ++m) ext[(size_t)m * eplane + eoff] = c[m];
   }
+}
+
+// Both sweep directions in one launch: blocks [0, nbx) take the x-sweep lines, the rest the
+// y-sweep columns.  The two are independent (each writes its own planes), so they overlap on the
+// device instead of running as two dependent launches.
+__global__ void __launch_bounds__(64 * kWPBA) adj_sweeps(AdjArgs a, int nbx) {
+  const int b = (int)blockIdx.x;
+  if (b < nbx) sweep_body<0>(a, b, nbx);
+  else sweep_body<1>(a, b - nbx, (int)gridDim.x - nbx);
 }
 
 // extended index k (0..4 = i, -2, -1, n, n+1) along one axis: does its value come from cell i, and
@@ -217,10 +227,11 @@ __global__ void __launch_bounds__(256) adj_fold(AdjArgs a) {
   double acc[3];
 #pragma unroll
   for (int m = 0; m < 3; ++m) acc[m] = a.lam[(size_t)m * plane + t];
+  const double* const ey_ = a.ext + 3 * eplane;  // the y-sweep's planes
   if (i >= 2 && i < a.mx - 2 && j >= 2 && j < a.my - 2) {
     const size_t eo = (size_t)(j + 2) * ew + (size_t)(i + 2);
 #pragma unroll
-    for (int m = 0; m < 3; ++m) acc[m] += a.ext[(size_t)m * eplane + eo];
+    for (int m = 0; m < 3; ++m) acc[m] += a.ext[(size_t)m * eplane + eo] + ey_[(size_t)m * eplane + eo];
   } else {
     // fixed order: y candidate outer, x candidate inner
 #pragma unroll
@@ -236,7 +247,7 @@ __global__ void __launch_bounds__(256) adj_fold(AdjArgs a) {
         const size_t eo = (size_t)(ey + 2) * ew + (size_t)(ex + 2);
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-          const double v = a.ext[(size_t)m * eplane + eo];
+          const double v = a.ext[(size_t)m * eplane + eo] + ey_[(size_t)m * eplane + eo];
           acc[m] += ((fx && m == a.rfx) != (fy && m == a.rfy)) ? -v : v;
         }
       }
@@ -266,7 +277,7 @@ extern "C" {
 
 size_t claw2d_adjoint_workspace_bytes(const claw2d_desc* d) {
   if (check_adjoint_desc(d) != 0) return 0;
-  return (size_t)3 * (size_t)(d->mx + 4) * (size_t)(d->my + 4) * sizeof(double);
+  return (size_t)6 * (size_t)(d->mx + 4) * (size_t)(d->my + 4) * sizeof(double);
 }
 
 int claw2d_adjoint_step(const claw2d_desc* d, const void* q, const void* lam_in, void* lam_out, void* workspace,
@@ -301,8 +312,8 @@ int claw2d_adjoint_step(const claw2d_desc* d, const void* q, const void* lam_in,
   const long long wx = (long long)(d->my + 4) * a.nseg_x, wy = (long long)(d->mx + 4) * a.nseg_y;
   const long long cells = (long long)d->mx * d->my;
   if (wx > (1LL << 30) || wy > (1LL << 30)) return set_error(-23, "grid too large for the adjoint launch");
-  hipLaunchKernelGGL(adj_sweep<0>, dim3((unsigned)((wx + kWPBA - 1) / kWPBA)), dim3(64 * kWPBA), 0, stream, a);
-  hipLaunchKernelGGL(adj_sweep<1>, dim3((unsigned)((wy + kWPBA - 1) / kWPBA)), dim3(64 * kWPBA), 0, stream, a);
+  const int nbx = (int)((wx + kWPBA - 1) / kWPBA), nby = (int)((wy + kWPBA - 1) / kWPBA);
+  hipLaunchKernelGGL(adj_sweeps, dim3((unsigned)(nbx + nby)), dim3(64 * kWPBA), 0, stream, a, nbx);
   hipLaunchKernelGGL(adj_fold, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, stream, a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_hip_error(e, "adjoint launch");

@@ -172,6 +172,8 @@ template <class T, class P> struct StepArgs {
   int abx_lo, abx_hi, aby_lo, aby_hi;
   // the capacity plane kappa (aux plane mcapa-1) of a capacity-form step, else null
   const T* capa;
+  // launch shape: wavefronts per workgroup (1, 2 or 4) and XCD-contiguous order (1) or not (0)
+  int wpb, xcd;
   typename P::Par par;
 };
 
@@ -303,8 +305,11 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   static_assert(SWEEP == 0 || (SPLIT && PAD), "single sweeps run on the padded split path");
   static_assert(!(CAPA && PAD), "the capacity form runs on the index-mapped (unpadded) path");
   constexpr bool DPP = kDppShift<T, P>;
-  // zero-filled edges pay for the Euler kernels (FP32 -4%) but not for shallow water (FP64 +0.7%)
-  constexpr bool DPPZ = P::NEQ == 4;
+  // Zero-filled wavefront-shift edges (no copy pre-loads the destination): lanes 0 and 63 then
+  // receive 0 from outside the wavefront.  Only halo data sees it: owned cells (lanes 2..61) take
+  // the interfaces at lanes 2..62, whose inputs come from lanes 1..63 (measured on MI355X: FP64
+  // shallow water -3.7% together with the one-way limiter exchange below, Euler FP32 -4%).
+  constexpr bool DPPZ = true;
   constexpr int NEQ = P::NEQ, NW = P::NW, NC = P::NC, NL = P::NL;
   // 4 equations: add each y-interface's upper-cell shares at once (two more exchanges per row)
   // instead of carrying them to the next row, so the kernel fits 2 wavefronts per SIMD
@@ -376,6 +381,15 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   auto cfl_speed = [&](const Iface& f, T rkl, T rkr) -> T {
     if constexpr (CAPA) return vmax((f.u + f.a) * rkr, (f.a - f.u) * rkl);
     else return P::smax(f);
+  };
+  // The split step's y-interfaces see q*, which a rejected attempt (too large a dt) can leave with
+  // a negative pressure where the velocity is still finite: the sound speed a is then NaN and the
+  // extreme speeds u -+ a with it.  The reference's maximum runs over every wave and skips NaN
+  // speeds, so the middle waves (speed u) still count there; this adds them (they never exceed
+  // the extremes when a is a number, so the result is otherwise the same bits).
+  auto cfl_speed_q_star = [&](const Iface& f, T rkl, T rkr) -> T {
+    const T mid = CAPA ? vmax(f.u * rkr, -f.u * rkl) : vabs(f.u);
+    return vmax(cfl_speed(f, rkl, rkr), mid);
   };
   T rkprev = T(1), rkm2 = T(1);  // 1/kappa of rows R-1 and R-2 (capacity form)
 
@@ -450,7 +464,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         // in an accepted state is still caught by the stored-cell check)
         if ((SPLIT ? own_col : ycfl_col) && (ST || (R >= 0 && R <= my)))
         {
-          const T sp = cfl_speed(fynew, rkprev, rkR);
+          const T sp = SPLIT ? cfl_speed_q_star(fynew, rkprev, rkR) : cfl_speed(fynew, rkprev, rkR);
           if constexpr (SPLIT) smy = vmax(smy, sp);
           else if constexpr (kNanFlag) { smy = vmax(smy, sp); nany |= sp != sp; }
           else smy = nmax(smy, sp);
@@ -500,7 +514,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
                 qn[m] = CAPA ? qm2[m] + acc0[m] * rkm2 : qm2[m] + acc0[m];
                 qout[(size_t)m * plane + off] = qn[m];
               }
-              bad |= P::bad(qn);
+              bad |= P::bad(qn, a.par);
             }
           }
           if constexpr (!kImmY) {
@@ -535,22 +549,21 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         if constexpr (kNanFlag) { smx = vmax(smx, sp); nanx |= sp != sp; }
         else smx = nmax(smx, sp);
       }
-      // limiter partners: both neighbours' wave data in one exchange round; the left pair's
-      // eigenvector products are recomputed here (the same operands in the same order as on the
-      // left lane, so the same bits)
-      T lself[NL], lnb[NL], lleft[NL], mR[NW], mL[NW], alR[NW], alL[NW];
+      // limiter partners.  The right neighbour's wave data come over one exchange; this lane
+      // forms the eigenvector products m_p of its pair (this, right) and hands the right
+      // neighbour the product it needs when its wave p is right-going, alpha_p(this) * m_p, which
+      // is its left partner's upwind term (NW values instead of the left partner's NL).
+      T lself[NL], lnb[NL], mR[NW], mL[NW], alR[NW], alL[NW];
       P::lim(fxi, lself);
 #pragma unroll
-      for (int k = 0; k < NL; ++k) {
-        lnb[k] = from_right<DPP, T, DPPZ>(lself[k]);
-        lleft[k] = from_left<DPP, T, DPPZ>(lself[k]);
-      }
+      for (int k = 0; k < NL; ++k) lnb[k] = from_right<DPP, T, DPPZ>(lself[k]);
       P::mdot(lself, lnb, mR);
-      P::mdot(lleft, lself, mL);
 #pragma unroll
       for (int p = 0; p < NW; ++p) {
+        const T up = P::unit_m(p) ? lself[p] : lself[p] * mR[p];
+        alL[p] = from_left<DPP, T, DPPZ>(up);
+        mL[p] = T(1);
         alR[p] = lnb[p];
-        alL[p] = lleft[p];
       }
       T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
       finish_iface<T, P, LIM, TWC, CAPA>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2,
@@ -743,7 +756,8 @@ step2_fused(StepArgs<T, P> a) {
     if (a.ctrl->cur) { qin = a.q1; qout = a.q0; }
   }
   // wave-uniform quantities live in SGPRs (readfirstlane), so the row loop branches on SCC
-  const int wave = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * kWPB + (threadIdx.x >> 6));
+  const int blk = a.xcd ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(blk * a.wpb + (threadIdx.x >> 6));
   if (wave >= a.nstrips * a.nsegs) return;
   const int strip = wave % a.nstrips, seg = wave / a.nstrips;
   const int x0 = strip * kOwn, y0 = seg * a.seg_h;
@@ -862,13 +876,16 @@ __global__ void __launch_bounds__(kSumThreads) mass_partial(const T* q, size_t n
   }
 }
 
-__global__ void __launch_bounds__(64) mass_final(const double* part, double* out) {
+// offset: a constant added once to the sum (the reference depth times the cell count of a state
+// that stores h - depth_ref), as one more compensated term
+__global__ void __launch_bounds__(64) mass_final(const double* part, double* out, double offset) {
   if (threadIdx.x != 0) return;
   double s = 0.0, comp = 0.0;
   for (int b = 0; b < kSumBlocks; ++b) {
     comp += part[2 * b + 1];
     two_sum_acc(s, comp, part[2 * b]);
   }
+  if (offset != 0.0) two_sum_acc(s, comp, offset);
   out[0] = s + comp;
 }
 
@@ -881,13 +898,18 @@ __global__ void __launch_bounds__(64) mass_final(const double* part, double* out
 // more (FP32 split: 6) gets shorter segments from the full count, and at 4096^2 their extra halo
 // rows cost more than the occupancy returns (FP32 split: 0.241 ms at 6 per SIMD, 0.220 at 4).
 constexpr long kMaxWavesPerCU = 16;
-static long resident_waves(const void* kern) {
+static long resident_waves(const void* kern, int wpb = kWPB) {
   int dev = 0, cus = 0, bpc = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, kLanes * kWPB, 0) != hipSuccess) return 0;
-  return (long)cus * std::min<long>((long)bpc * kWPB, kMaxWavesPerCU);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, kLanes * wpb, 0) != hipSuccess) return 0;
+  return (long)cus * std::min<long>((long)bpc * wpb, kMaxWavesPerCU);
 }
+
+// wavefronts per workgroup of a single-grid launch (desc->wg_waves; 0 = kWPB)
+static int launch_wpb(const claw2d_desc* d) { return d->wg_waves > 0 ? d->wg_waves : kWPB; }
+// occupancy cache index of a wavefronts-per-workgroup count 1, 2 or 4
+static int wpb_index(int wpb) { return wpb == 1 ? 0 : (wpb == 2 ? 1 : 2); }
 
 // SIMDs of the current device (4 per CU), 0 if the query fails.
 static long device_simds() {
@@ -975,6 +997,12 @@ static int check_desc(const claw2d_desc* d) {
       return fail(-16, "this Riemann solver reads no aux arrays: num_aux > 0 needs a capacity function (mcapa)");
     if (d->fwave != 0) return fail(-17, "fwave = 1 needs an f-wave Riemann solver (shallow_bathymetry_fwave)");
   }
+  if (d->wg_waves != 0 && d->wg_waves != 1 && d->wg_waves != 2 && d->wg_waves != 4)
+    return fail(-36, "wg_waves must be 0 (default), 1, 2 or 4");
+  if (d->launch_order != 0 && d->launch_order != 1) return fail(-37, "launch_order must be 0 or 1");
+  if (!std::isfinite(d->depth_ref)) return fail(-35, "depth_ref must be finite");
+  if (d->depth_ref != 0.0 && (d->solver != CLAW_SOLVER_SHALLOW_ROE_EFIX || d->dtype != CLAW_FP32))
+    return fail(-35, "depth_ref (stored depth h - depth_ref) is for FP32 shallow_roe_with_efix states only");
   if (d->num_aux > 0) {
     for (int k = 0; k < 2; ++k) {
       const int lo = d->aux_bc_lower[k], hi = d->aux_bc_upper[k];
@@ -989,6 +1017,8 @@ static int check_desc(const claw2d_desc* d) {
 
 template <class T, class P>
 static void fill_args(StepArgs<T, P>& a, const claw2d_desc* d, long resident) {
+  a.wpb = launch_wpb(d);
+  a.xcd = d->launch_order == 0 ? 1 : 0;
   a.dx = d->dx;
   a.dy = d->dy;
   a.mx = d->mx;
@@ -1017,6 +1047,7 @@ template <class T> static typename SweP<T>::Par swe_par(const claw2d_desc* d) {
   typename SweP<T>::Par p;
   p.g = T(d->grav);
   p.sqg = T(std::sqrt(d->grav));
+  p.h0 = T(d->depth_ref);
   return p;
 }
 template <class T> static typename SweBathyP<T>::Par bathy_par(const claw2d_desc* d) {
@@ -1073,10 +1104,13 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   } else if (!pad || SPLIT) {
     kern = step_kernel<T, P, SPLIT>(d, &slot);
   }
-  static thread_local long cached[7] = {-1, -1, -1, -1, -1, -1, -1};
-  if (cached[slot] < 0) cached[slot] = resident_waves(kern);
+  const int wpb = launch_wpb(d);
+  static thread_local long cached[7][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1},
+                                           {-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};
+  long& res = cached[slot][wpb_index(wpb)];
+  if (res < 0) res = resident_waves(kern, wpb);
   StepArgs<T, P> a;
-  fill_args(a, d, cached[slot]);
+  fill_args(a, d, res);
   a.q0 = (T*)q0;
   a.q1 = (T*)q1;
   a.ctrl = ctrl;
@@ -1092,9 +1126,9 @@ static int launch_step(const claw2d_desc* d, const typename P::Par& par, void* q
   if (d->mcapa > 0) a.capa = (const T*)aux + (size_t)(d->mcapa - 1) * (size_t)d->mx * (size_t)d->my;
   const int waves = a.nstrips * a.nsegs;
   a.total_waves = waves;
-  const int blocks = (waves + kWPB - 1) / kWPB;
+  const int blocks = (waves + wpb - 1) / wpb;
   void* args[] = {&a};
-  const hipError_t e = hipLaunchKernel(kern, dim3(blocks), dim3(kLanes * kWPB), args, 0, s);
+  const hipError_t e = hipLaunchKernel(kern, dim3(blocks), dim3(kLanes * wpb), args, 0, s);
   return e == hipSuccess ? 0 : hip_fail(e, "step2_fused launch");
 }
 
@@ -1106,9 +1140,10 @@ static int launch_sweep(const claw2d_desc* d, const typename P::Par& par, const 
                         const void* aux, claw2d_ctrl* ctrl, double dt, int ids, hipStream_t s) {
   const void* kern = ids == 1 ? (const void*)step2_fused<T, P, 0, true, -1, true, 1>
                               : (const void*)step2_fused<T, P, 0, true, -1, true, 2>;
-  static thread_local long cached[2] = {-1, -1};
-  long& res = cached[ids - 1];
-  if (res < 0) res = resident_waves(kern);
+  const int wpb = launch_wpb(d);
+  static thread_local long cached[2][3] = {{-1, -1, -1}, {-1, -1, -1}};
+  long& res = cached[ids - 1][wpb_index(wpb)];
+  if (res < 0) res = resident_waves(kern, wpb);
   StepArgs<T, P> a;
   fill_args(a, d, res);
   a.q0 = (T*)qbc;
@@ -1125,9 +1160,9 @@ static int launch_sweep(const claw2d_desc* d, const typename P::Par& par, const 
   a.aux = (const T*)aux;
   const int waves = a.nstrips * a.nsegs;
   a.total_waves = waves;
-  const int blocks = (waves + kWPB - 1) / kWPB;
+  const int blocks = (waves + wpb - 1) / wpb;
   void* args[] = {&a};
-  const hipError_t e = hipLaunchKernel(kern, dim3(blocks), dim3(kLanes * kWPB), args, 0, s);
+  const hipError_t e = hipLaunchKernel(kern, dim3(blocks), dim3(kLanes * wpb), args, 0, s);
   return e == hipSuccess ? 0 : hip_fail(e, "step2_fused sweep launch");
 }
 
@@ -1423,7 +1458,8 @@ int claw2d_mass_sum(const claw2d_desc* d, const void* q, int32_t m, double* out,
   else
     hipLaunchKernelGGL(mass_partial<float>, dim3(kSumBlocks), dim3(kSumThreads), 0, s,
                        (const float*)q + (size_t)m * n, n, (double*)ws);
-  hipLaunchKernelGGL(mass_final, dim3(1), dim3(64), 0, s, (const double*)ws, out);
+  hipLaunchKernelGGL(mass_final, dim3(1), dim3(64), 0, s, (const double*)ws, out,
+                     m == 0 ? d->depth_ref * (double)n : 0.0);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : hip_fail(e, "mass_sum launch");
 }

@@ -99,24 +99,37 @@ template <class T> __device__ __forceinline__ T limited(T al, T t, int meth) {
 // Shallow water, Roe + entropy fix.  Waves e1 = (1, u-a, v), e2 = (0, 0, 1), e3 = (1, u+a, v);
 // transverse eigenvectors f1 = (1, u, v-a), f2 = (0, 1, 0), f3 = (1, u, v+a).
 // ------------------------------------------------------------------------------------------
+//
+// Stored depth (FP32 only).  The FP32 state stores the depth as its deviation eta = h - h0 from a
+// reference depth h0 (Par::h0, the descriptor's depth_ref; 0 stores h itself).  Rounding the
+// stored depth to FP32 is the dominant error of the FP32 path, and it is relative to the stored
+// value: storing eta, which is small where the water is near h0, makes it that much smaller,
+// while the jumps across interfaces (the wave strengths) are the same differences of stored
+// values.  Wherever the depth itself enters (square roots, the Roe average, the transonic tests)
+// it is formed as h0 + eta.  The FP64 state always stores h (h0 is not read).
 template <class T> struct SweP {
   static constexpr int NEQ = 3, NW = 3, NC = 7, NL = 6;
   static constexpr int NAUX = 0;        // aux fields read per cell
   static constexpr bool FWAVE = false;  // waves (not f-waves)
+  static constexpr bool kRef = sizeof(T) == 4;  // stored depth is h - h0
   // e_p(A).e_p(B) is identically 1 for the shear wave
   __device__ __forceinline__ static constexpr bool unit_m(int p) { return p == 1; }
-  struct Par { T g, sqg; };
-  // per-cell: q, sqrt(h), 1/sqrt(h), hu/sqrt(h), hv/sqrt(h)
+  struct Par { T g, sqg, h0; };
+  // per-cell: q as stored, sqrt(h), 1/sqrt(h), hu/sqrt(h), hv/sqrt(h)
   struct Cell { T q[3]; T s, rs, P, Q; };
   struct Iface { T al[3]; T sf[3]; T u, v, a, ha; };
 
-  __device__ __forceinline__ static bool bad(const T q[3]) { return !(q[0] > T(0)); }
+  // depth from a stored depth value
+  __device__ __forceinline__ static T depth(T stored, const Par& pr) { return kRef ? stored + pr.h0 : stored; }
 
-  __device__ __forceinline__ static Cell prep(const T q[3], const Par&) {
+  __device__ __forceinline__ static bool bad(const T q[3], const Par& pr) { return !(depth(q[0], pr) > T(0)); }
+
+  __device__ __forceinline__ static Cell prep(const T q[3], const Par& pr) {
     Cell c;
     c.q[0] = q[0]; c.q[1] = q[1]; c.q[2] = q[2];
-    c.rs = frsq(q[0]);
-    c.s = q[0] * c.rs;
+    const T h = depth(q[0], pr);
+    c.rs = frsq(h);
+    c.s = h * c.rs;
     c.P = q[1] * c.rs;
     c.Q = q[2] * c.rs;
     return c;
@@ -138,7 +151,7 @@ template <class T> struct SweP {
     const T inv = frcp(L.s + R.s);
     const T u = (pnL + pnR) * inv;
     const T v = (ptL + ptR) * inv;
-    const T gh = pr.g * ((L.q[0] + R.q[0]) * T(0.5));
+    const T gh = kRef ? pr.g * (pr.h0 + (L.q[0] + R.q[0]) * T(0.5)) : pr.g * ((L.q[0] + R.q[0]) * T(0.5));
     const T ra = frsq(gh);
     const T a = gh * ra;
     const T ha = T(0.5) * ra;
@@ -150,7 +163,7 @@ template <class T> struct SweP {
     T sf1 = vmin(s1, T(0)), sf2 = T(0), sf3 = T(0);
     const T s0 = pnL * L.rs - pr.sqg * L.s;  // u - c in the left cell
     if (s0 < T(0)) {
-      const T h1 = L.q[0] + a1, hu1 = L.q[mu] + a1 * s1;
+      const T h1 = depth(L.q[0] + a1, pr), hu1 = L.q[mu] + a1 * s1;
       if (h1 > T(0) && hu1 > T(0) && hu1 * hu1 > pr.g * (h1 * h1) * h1) {
         const T s1r = hu1 / h1 - sqrt(pr.g * h1);  // u - c right of the 1-wave
         sf1 = s0 * (s1r - s1) / (s1r - s0);
@@ -159,7 +172,7 @@ template <class T> struct SweP {
     if (u < T(0)) {
       sf2 = u;
       sf3 = vmin(s3, T(0));
-      const T h3 = R.q[0] - a3, hu3 = R.q[mu] - a3 * s3;
+      const T h3 = depth(R.q[0] - a3, pr), hu3 = R.q[mu] - a3 * s3;
       if (h3 > T(0) && hu3 < T(0) && hu3 * hu3 > pr.g * (h3 * h3) * h3) {
         const T s3r = pnR * R.rs + pr.sqg * R.s;  // u + c in the right cell
         if (s3r > T(0)) {
@@ -256,7 +269,7 @@ template <class T> struct EulerP {
   struct Cell { T q[4]; T rs, P, Q, Hs, c; };
   struct Iface { T al[4]; T sf[4]; T u, v, a, H, i2a; };
 
-  __device__ __forceinline__ static bool bad(const T q[4]) { return !(q[0] > T(0)); }
+  __device__ __forceinline__ static bool bad(const T q[4], const Par&) { return !(q[0] > T(0)); }
 
   __device__ __forceinline__ static Cell prep(const T q[4], const Par& pr) {
     Cell c;
@@ -281,12 +294,18 @@ template <class T> struct EulerP {
     return c;
   }
 
-  // squared-form test of "n/rho - c > 0" (right = true) or "n/rho + c < 0" (right = false) for
-  // the state (rho, n, t, E), using p*rho = gamma1*(E*rho - (n^2+t^2)/2).
+  // Squared-form test of "n/rho - c > 0" (right = true) or "n/rho + c < 0" (right = false) for
+  // the state (rho, n, t, E) with c = sqrt(gamma p / rho), as upstream evaluates it.  A middle state
+  // of the Roe split can have rho < 0 together with p < 0; p/rho > 0 there and c is real, so
+  // upstream's test can pass, and the fix is applied.  With Prho = p rho = gamma1 (E rho -
+  // (n^2 + t^2)/2), c = sqrt(gamma Prho)/|rho| is real iff Prho >= 0, and for either sign of rho
+  //   n/rho - c > 0  <=>  n/rho > 0 and n^2 > gamma Prho,
+  //   n/rho + c < 0  <=>  n/rho < 0 and n^2 > gamma Prho.
   __device__ __forceinline__ static bool sonic_cross(T rho, T n, T t, T E, const Par& pr, bool right) {
     const T Prho = pr.gamma1 * (E * rho - T(0.5) * (n * n + t * t));
-    const bool dir = right ? (n > T(0)) : (n < T(0));
-    return rho > T(0) && dir && Prho > T(0) && n * n > pr.gamma * Prho;
+    const bool pos = (n > T(0) && rho > T(0)) || (n < T(0) && rho < T(0));  // n/rho > 0
+    const bool neg = (n < T(0) && rho > T(0)) || (n > T(0) && rho < T(0));  // n/rho < 0
+    return (right ? pos : neg) && Prho >= T(0) && n * n > pr.gamma * Prho;
   }
   __device__ __forceinline__ static T sonic_speed(T rho, T n, T t, T E, const Par& pr, T sign) {
     const T p = pr.gamma1 * (E - T(0.5) * (n * n + t * t) / rho);
@@ -464,7 +483,7 @@ template <class T> struct SweBathyP : SweP<T> {
   using Iface = typename SweP<T>::Iface;
 
   // depth 0 is a dry cell, a negative depth (or NaN) is a bad state
-  __device__ __forceinline__ static bool bad(const T q[3]) { return !(q[0] >= T(0)); }
+  __device__ __forceinline__ static bool bad(const T q[3], const Par&) { return !(q[0] >= T(0)); }
 
   __device__ __forceinline__ static Cell prep(const T q[3], T b, const Par& pr) {
     Cell c;

@@ -24,12 +24,19 @@ def make_desc(solver_id, dtype, mx, my, dx, dy, num_eqn, num_waves, order=2, tra
               mthlim=(4, 4, 4, 4), bc_lower=(1, 1), bc_upper=(1, 1), reflect_index=(1, 2),
               grav=1.0, gamma=1.4, cfl_max=1.0, cfl_desired=0.9, dt_max=1e99, dt_variable=True,
               seg_h=0, num_ghost=2, num_aux=None, fwave=None, aux_bc_lower=None, aux_bc_upper=None,
-              dry_tolerance=1e-3, index_capa=-1):
+              dry_tolerance=1e-3, index_capa=-1, depth_ref=0.0, wg_waves=0, launch_order=0):
     """num_aux and fwave default to what the solver needs (one aux field, the bathymetry, and
     f-waves for shallow_bathymetry_fwave; with a capacity function at least index_capa + 1 aux
     fields); aux BCs default to the state's.  index_capa >= 0: aux[index_capa] is the capacity
-    function kappa (pyclaw's state.index_capa), the step is the capacity form."""
+    function kappa (pyclaw's state.index_capa), the step is the capacity form.  depth_ref (FP32
+    shallow water only): the device state stores the depth as h - depth_ref; upload, download and
+    mass_sum convert, the raw planes (planes(), set_planes(), step_into) hold the stored form.
+    wg_waves (1, 2, 4; 0 = 4) and launch_order (0 XCD-contiguous, 1 plain) set the launch shape
+    of the single-grid kernels, as seg_h sets the rows per wavefront; results do not depend on
+    them."""
     d = ClawDesc()
+    d.depth_ref = float(depth_ref)
+    d.wg_waves, d.launch_order = int(wg_waves), int(launch_order)
     bathy = int(solver_id) == _lib.SOLVER_SHALLOW_BATHY_FWAVE
     if num_aux is None:
         num_aux = max(1 if bathy else 0, int(index_capa) + 1)
@@ -134,8 +141,11 @@ class DeviceGrid:
 
     # ---- state transfer -------------------------------------------------------------------
     def upload(self, q_host):
-        """q_host: pyclaw layout (num_eqn, mx, my) -> device planes (num_eqn, my, mx)."""
-        q = torch.from_numpy(np.ascontiguousarray(np.asarray(q_host, dtype=np.float64)))
+        """q_host: pyclaw layout (num_eqn, mx, my) -> device planes (num_eqn, my, mx); with a
+        depth_ref the depth is stored as h - depth_ref (formed in FP64, then rounded)."""
+        q = torch.from_numpy(np.array(q_host, dtype=np.float64))
+        if self.desc.depth_ref != 0.0:
+            q[0] -= self.desc.depth_ref
         planes = q.permute(0, 2, 1).contiguous().to(self.torch_dtype)
         self.q[self.cur].copy_(planes)
 
@@ -147,8 +157,11 @@ class DeviceGrid:
         return self.q[self.cur]
 
     def download(self):
-        """Current state in pyclaw layout (num_eqn, mx, my), float64 numpy, Fortran order."""
+        """Current state in pyclaw layout (num_eqn, mx, my), float64 numpy, Fortran order (a
+        stored depth deviation is turned back into h in FP64)."""
         t = self.q[self.cur].to(torch.float64).cpu()
+        if self.desc.depth_ref != 0.0:
+            t[0] += self.desc.depth_ref
         return np.asfortranarray(t.permute(0, 2, 1).numpy())
 
     # ---- compute --------------------------------------------------------------------------

@@ -95,8 +95,14 @@ class ClawSolver2D(Solver):
         self.kernel_language = "HIP"
         self.reflect_index = [1, 2]
         self.precision = "fp64"
+        # FP32 shallow water: the device stores the depth as h - depth_ref (its FP32 rounding is
+        # then relative to the deviation, not to h).  None: the mean initial depth, rounded to
+        # FP32, when no user code sees the device state (no step_source, before_step or
+        # BC.custom), else 0; a number sets it; 0 stores h.
+        self.depth_ref = None
+        self._auto_depth_ref = 0.0
         self.seg_h = 0          # rows per marching segment; 0 = occupancy-derived default
-        self.autotune = False   # True: seg_h from clawpack_amd.autotune (measured, cached)
+        self.autotune = False   # True: tile shape (seg_h, wg_waves, launch_order) from clawpack_amd.autotune
         self._grid = None
         self._desc_key = None
 
@@ -204,12 +210,21 @@ class ClawSolver2D(Solver):
             seg_h=self.seg_h, num_aux=state.num_aux, fwave=bool(self.fwave),
             aux_bc_lower=[b if b is not None else 1 for b in self.aux_bc_lower],
             aux_bc_upper=[b if b is not None else 1 for b in self.aux_bc_upper],
-            dry_tolerance=pd.get("dry_tolerance", 0.0), index_capa=state.index_capa)
+            dry_tolerance=pd.get("dry_tolerance", 0.0), index_capa=state.index_capa,
+            depth_ref=self._depth_ref(pol))
+
+    def _depth_ref(self, pol):
+        if self.precision != "fp32" or pol.kernel_id != _lib.SOLVER_SHALLOW_ROE_EFIX:
+            return 0.0
+        if self.step_source is not None or self.before_step is not None or self._has_custom_bc():
+            return 0.0  # user functions see the device planes, which then hold h itself
+        return float(self._auto_depth_ref if self.depth_ref is None else self.depth_ref)
 
     def setup(self, solution):
         state = solution.state
         self._check_supported(state)
         self._check_cfl_settings()
+        self._auto_depth_ref = float(np.float32(np.mean(state.q[0]))) if state.q.size else 0.0
         if self.dt is None:
             self.dt = self.dt_initial
         self._device(state)
@@ -219,7 +234,7 @@ class ClawSolver2D(Solver):
         desc = self._make_desc(state)
         if self.autotune and self.seg_h == 0:
             from .. import autotune
-            desc.seg_h = autotune.tuned_seg_h(desc, split=bool(self.dimensional_split))
+            autotune.apply_shape(desc, autotune.tuned_shape(desc, split=bool(self.dimensional_split)))
         key = (desc.mx, desc.my, desc.num_eqn, desc.dtype, desc.num_aux)
         if self._grid is None or self._desc_key != key:
             self._grid = DeviceGrid(desc)

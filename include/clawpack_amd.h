@@ -88,6 +88,18 @@ typedef struct claw2d_desc {
    * and adjoint entry points reject it. */
   int32_t mcapa;
   int32_t reserved_capa;
+  /* Reference depth of an FP32 shallow-water state (CLAW_SOLVER_SHALLOW_ROE_EFIX, CLAW_FP32):
+   * its depth plane holds h - depth_ref, which the kernels add back wherever the depth itself
+   * enters.  FP32 rounding of the stored depth is then relative to the deviation, not to h (the
+   * dominant error of the FP32 path).  0 = the plane holds h; FP64 states and the other solvers
+   * take only 0.  claw2d_mass_sum returns the sum of h (depth_ref times the cell count added). */
+  double depth_ref;
+  /* Launch shape of the single-grid step kernels (results do not depend on it): wavefronts per
+   * workgroup, 1, 2 or 4 (0 = 4), and the order in which workgroups take strips and segments,
+   * 0 = each XCD takes a contiguous range (neighbouring strips share halo lines in one L2),
+   * 1 = plain order (workgroup b to the XCD b mod 8, as dispatched). */
+  int32_t wg_waves;
+  int32_t launch_order;
 } claw2d_desc;
 
 /* Device-resident time-loop control block (one per grid). */
@@ -191,8 +203,9 @@ int claw2d_sweep_padded(const claw2d_desc* d, const void* qbc_in_dev, void* q_ou
 size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc* d);
 
 /* Compensated (Neumaier) FP64 sum of component m over the interior of q_dev (FP32 or FP64 state),
- * in a fixed reduction order (bitwise reproducible).  Writes the sum to out_dev[0].  No
- * counterpart in the reference (pyclaw sums with numpy in FP64). */
+ * in a fixed reduction order (bitwise reproducible).  Writes the sum to out_dev[0]; for m = 0 of a
+ * state with a depth_ref, the sum of h (depth_ref times mx*my added as one more compensated term).
+ * No counterpart in the reference (pyclaw sums with numpy in FP64). */
 int claw2d_mass_sum(const claw2d_desc* d, const void* q_dev, int32_t m, double* out_dev,
                     void* workspace_dev, void* stream);
 
@@ -200,14 +213,14 @@ int claw2d_mass_sum(const claw2d_desc* d, const void* q_dev, int32_t m, double* 
  * Adjoint (no counterpart in the reference: PyClaw has no adjoint).  Shallow water, FP64.
  * ---------------------------------------------------------------------------------------- */
 
-/* Bytes of the workspace claw2d_adjoint_step needs: 3*(mx+4)*(my+4) doubles. */
+/* Bytes of the workspace claw2d_adjoint_step needs: 6*(mx+4)*(my+4) doubles. */
 size_t claw2d_adjoint_workspace_bytes(const claw2d_desc* d);
 
 /* Reverse of one fixed-dt step: lam_out = (d q^{n+1} / d q^n)^T lam_in, where q^{n+1} is what
  * claw2d_step makes from q_dev = q^n with this dt (same method, limiters, BCs, physics).  All
  * arrays are SoA planes (3, my, mx) of doubles in device memory; lam_out may alias lam_in.
  * Branches (entropy fix, limiter case, upwind side, transverse speed signs) are those the forward
- * takes at q^n.  Three kernels on `stream`; the workspace is overwritten. */
+ * takes at q^n.  Two kernels on `stream`; the workspace is overwritten. */
 int claw2d_adjoint_step(const claw2d_desc* d, const void* q_dev, const void* lam_in_dev,
                         void* lam_out_dev, void* workspace_dev, double dt, void* stream);
 

@@ -87,7 +87,7 @@ def test_adjoint_workspace_and_validation():
     from clawpack_amd import _lib
     lib = _lib.load()
     d = _desc()
-    assert lib.claw2d_adjoint_workspace_bytes(C.byref(d)) == 3 * (32 + 4) * (16 + 4) * 8
+    assert lib.claw2d_adjoint_workspace_bytes(C.byref(d)) == 6 * (32 + 4) * (16 + 4) * 8
     # Euler and FP32 have no adjoint: size 0 and a message
     for bad in (_desc(solver_id=1, num_eqn=4, num_waves=4), _desc(dtype=1)):
         assert lib.claw2d_adjoint_workspace_bytes(C.byref(bad)) == 0
@@ -209,3 +209,17 @@ def test_capacity_descriptor_rules():
     assert lib.claw2d_adjoint_workspace_bytes(C.byref(ok)) == 0
     assert lib.claw2d_batch_advance(C.byref(ok), qa, 1, 1, qa, qb, ctrl, aux, 1, None) == -25
 
+
+
+@pytest.mark.parametrize("kw,ok", [
+    (dict(dtype=1, depth_ref=1.0), True), (dict(dtype=0, depth_ref=1.0), False),
+    (dict(dtype=1, depth_ref=float("nan")), False),
+    (dict(solver_id=1, num_eqn=4, num_waves=4, dtype=1, depth_ref=1.0), False),
+])
+def test_depth_ref_only_for_fp32_shallow_water(kw, ok):
+    """A stored depth deviation (h - depth_ref) is an FP32 shallow-water option."""
+    from clawpack_amd import _lib
+    lib = _lib.load()
+    d = _desc(**kw)
+    rc = lib.claw2d_validate(C.byref(d))
+    assert (rc == 0) if ok else (rc == -35), rc

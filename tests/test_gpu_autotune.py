@@ -1,5 +1,6 @@
-"""Autotuner: measures seg_h candidates, caches the winner, and the tuned decomposition gives the
-same result bit for bit (the work split never changes the arithmetic)."""
+"""Autotuner: measures tile shapes (rows per wavefront, wavefronts per workgroup, launch order),
+caches the winner, and every shape gives the same result bit for bit (the work split and launch
+shape never change the arithmetic)."""
 import json
 import os
 
@@ -18,23 +19,26 @@ def test_tune_and_cache(tmp_path, monkeypatch):
     n = 512
     q, dx, dy = problems.gaussian_humps(n, n, 1)
     d = make_desc(0, 0, n, n, dx, dy, 3, 3, bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0)
-    best, ms = autotune.tune_grid(d, q, 1e-3, candidates=(0, 16, 64, 128), steps=5)
-    print("tuned seg_h", best, ms)
-    assert str(best) in ms and ms[str(best)] == min(ms.values())
+    shape, ms = autotune.tune_grid(d, q, 1e-3, candidates=(0, 16, 64, 128), steps=5)
+    print("tuned shape", shape, ms)
+    name = autotune.shape_name(shape["seg_h"], shape["wg_waves"], shape["launch_order"])
+    assert name in ms and ms[name] == min(ms.values())
+    # every packing and order was measured at the best height
+    assert sum(1 for n in ms if n.startswith("h%d/" % shape["seg_h"])) >= len(autotune.SHAPES)
     table = json.load(open(os.environ["CLAWPACK_AMD_TUNE_CACHE"]))
-    assert list(table.values())[0]["seg_h"] == best
-    assert autotune.tuned_seg_h(d) == best  # served from the cache
+    assert list(table.values())[0]["seg_h"] == shape["seg_h"]
+    assert autotune.tuned_shape(d) == shape  # served from the cache
     outs = []
-    for seg in (0, best, 16):
-        dd = type(d).from_buffer_copy(bytes(d))
-        dd.seg_h = seg
-        g = DeviceGrid(dd)
+    for seg, wpb, order in ((0, 0, 0), (shape["seg_h"], shape["wg_waves"], shape["launch_order"]),
+                            (16, 1, 1), (24, 2, 0)):
+        g = DeviceGrid(autotune._with_shape(d, seg, wpb, order))
         g.upload(q)
         g.init_ctrl(0.0, 1e-3)
         for _ in range(3):
             g.enqueue_step(1e-3)
         outs.append(g.download())
-    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
 
 
 def test_tune_batch(tmp_path, monkeypatch):

@@ -101,21 +101,23 @@ def test_shallow_water_full_size_matches_oracle(n, split):
 
 
 def test_euler_2048_matches_oracle():
+    """Config 1 at its full size: 40 controller steps of the 2048^2 quadrant problem (the oracle
+    takes about 0.5 s per step on 16 threads)."""
     print("case full-size euler vs oracle", flush=True)
-    n = 2048
+    n, steps = 2048, 40
     q, dx, dy = problems.euler_quadrants(n, n, 1)
     dt0 = euler_dt(q, dx, dy)
     g = DeviceGrid(euler_desc(n, dx, dy), DEV)
     g.upload(q)
-    g.init_ctrl(0.0, dt0, target_steps=3)
-    g.advance(5)
+    g.init_ctrl(0.0, dt0, target_steps=steps)
+    g.advance(steps + 2)
     c = g.sync_cur()
     qg = g.download()
     print("  device done", flush=True)
     oracle_threads()
     od = corc.make_desc("euler", n, n, dx, dy, gamma=1.4)
-    qo, st = corc.evolve(od, corc.make_ctrl(), q, nsteps=3, dt=dt0)
-    assert c.numsteps == st["numsteps"] == 3 and c.rejected == st["rejected"]
+    qo, st = corc.evolve(od, corc.make_ctrl(), q, nsteps=steps, dt=dt0)
+    assert c.numsteps == st["numsteps"] == steps and c.rejected == st["rejected"]
     assert abs(c.t - st["t"]) <= 1e-12 * st["t"]
     err = float(np.abs(qg - qo).max()) / float(np.abs(q).max())
     assert err <= 1e-12, err

@@ -11,10 +11,10 @@ from oracle import corc
 pytestmark = pytest.mark.gpu
 
 
-def _grid(dtype, q, dx, dy, bc=(2, 2)):
+def _grid(dtype, q, dx, dy, bc=(2, 2), depth_ref=0.0):
     from clawpack_amd.device import DeviceGrid, make_desc
     neq, mx, my = q.shape
-    d = make_desc(0, dtype, mx, my, dx, dy, 3, 3, bc_lower=bc, bc_upper=bc, grav=1.0)
+    d = make_desc(0, dtype, mx, my, dx, dy, 3, 3, bc_lower=bc, bc_upper=bc, grav=1.0, depth_ref=depth_ref)
     g = DeviceGrid(d)
     g.upload(q)
     return g
@@ -123,3 +123,72 @@ def _oracle_times(od, c, q, steps):
             out.append(t)
         dt = min(c.dt_max, dt * c.cfl_desired / cfl)
     return np.array(out)
+
+
+def test_fp32_per_component_error_with_stored_depth_deviation():
+    """The FP32 path with the depth stored as h - 1 (depth_ref = 1, the humps' still-water level)
+    against the FP64 path after 1000 fixed-dt steps at 2048^2: every component, relative to its
+    own scale (L1 = sum|dq_m| / sum|q_m|, Linf = max|dq_m| / max|q_m|), within 1e-5.  The same run
+    storing h itself is reported beside it (its momentum error is the larger one)."""
+    import math
+    import torch
+    n, steps = 2048, 1000
+    print("case fp32 per component", n, steps, flush=True)
+    q, dx, dy = problems.gaussian_humps(n, n, 2024)
+    dt = 0.8 * min(dx, dy) / (math.sqrt(float(q[0].max())) + 0.5)
+    out = {}
+    for name, dtype, ref in (("fp64", 0, 0.0), ("fp32", 1, 1.0), ("fp32_h", 1, 0.0)):
+        g = _grid(dtype, q, dx, dy, depth_ref=ref)
+        g.init_ctrl(0.0, dt, target_steps=steps)
+        for _ in range(steps):
+            g.enqueue_step(dt)
+        c = g.read_ctrl()
+        assert c.status == 0 and c.cfl <= 1.0
+        p = g.planes().to(torch.float64).clone()
+        p[0] += ref
+        out[name] = p
+        del g
+    a = out["fp64"]
+    errs = {}
+    for name in ("fp32", "fp32_h"):
+        b = out[name]
+        errs[name] = [(float((b[m] - a[m]).abs().sum() / a[m].abs().sum()),
+                       float((b[m] - a[m]).abs().max() / a[m].abs().max())) for m in range(3)]
+        print("  %-7s L1/Linf per component: %s" % (name, " ".join("%.2e/%.2e" % e for e in errs[name])),
+              flush=True)
+    for l1, linf in errs["fp32"]:
+        assert l1 <= 1e-5 and linf <= 1e-5, errs["fp32"]
+    # the deviation storage is the better of the two on the momenta
+    assert max(e[1] for e in errs["fp32"][1:]) < max(e[1] for e in errs["fp32_h"][1:])
+
+
+def test_fp32_depth_ref_front_end_and_mass_sum():
+    """The PyClaw front end picks depth_ref (the mean initial depth) for FP32 shallow water; the
+    state it returns is h, and the device mass sum is the sum of h."""
+    from clawpack_amd import pyclaw, riemann
+    n = 96
+    q, dx, dy = problems.gaussian_humps(n, n, 7)
+    domain = pyclaw.Domain([pyclaw.Dimension(-2.5, 2.5, n, name="x"), pyclaw.Dimension(-2.5, 2.5, n, name="y")])
+    state = pyclaw.State(domain, 3)
+    state.problem_data["grav"] = 1.0
+    state.q[...] = q
+    solver = pyclaw.ClawSolver2D(riemann.shallow_roe_with_efix_2D)
+    solver.precision = "fp32"
+    solver.dimensional_split = False
+    solver.bc_lower = solver.bc_upper = [pyclaw.BC.periodic, pyclaw.BC.periodic]
+    sol = pyclaw.Solution(state, domain)
+    solver.setup(sol)
+    ref = solver._depth_ref(solver._policy())
+    assert ref == float(np.float32(q[0].mean())) and ref > 1.0
+    total0 = solver.total(sol, 0)
+    assert abs(total0 - q[0].sum()) <= 1e-6 * q[0].sum()
+    solver.evolve_to_time(sol, 0.2)
+    assert abs(solver.total(sol, 0) - total0) <= 2e-6 * total0
+    assert abs(state.q[0].mean() - q[0].mean()) <= 1e-6
+    # the front end's method: pyclaw's defaults, minmod and transverse_waves = 1
+    d64 = corc.make_desc("swe", n, n, dx, dy, grav=1.0, transverse_waves=1, mthlim=(1, 1, 1))
+    c = corc.make_ctrl(bc_lower=(2, 2), bc_upper=(2, 2), dimensional_split=False)
+    qo, _ = corc.evolve(d64, c, q, tend=0.2)
+    err = np.abs(state.q - qo).max() / np.abs(qo).max()
+    print("front end fp32 (depth_ref %.6f) vs oracle at t=0.2: %.2e" % (ref, err))
+    assert err <= 1e-5

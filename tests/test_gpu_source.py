@@ -153,13 +153,16 @@ def test_max_steps_spent_on_rejected_attempts(callbacks):
     """pyclaw raises "Maximum number of timesteps" only when all max_steps attempts were accepted
     short of tend.  The 200^2 dam break rejects its dt_initial = 0.1 attempt: with max_steps = 3
     it takes 2 steps and returns short of tend, on the device loop and on the host loop that a
-    before_step callback selects, like the oracle; a dt that is never rejected raises."""
+    before_step callback selects, like the oracle; dt_initial = 0.002, whose three attempts are
+    all accepted (the oracle's status says so), raises."""
     print("case max_steps callbacks", callbacks, flush=True)
     q0, dx, dy = problems.radial_dam_break(200, 200)
     d = corc.make_desc("swe", 200, 200, dx, dy, grav=1.0)
     _, st = corc.evolve(d, corc.make_ctrl(max_steps=3), q0, tend=1.0)
     assert st["error"] == 0 and st["numsteps"] == 2 and st["rejected"] == 1
-    for dt0, raises in ((0.1, False), (0.005, True)):
+    _, st2 = corc.evolve(d, corc.make_ctrl(max_steps=3, dt_initial=0.002), q0, tend=1.0)
+    assert st2["error"] == 1 and st2["numsteps"] == 3 and st2["rejected"] == 0
+    for dt0, raises in ((0.1, False), (0.002, True)):
         solver, sol, _, _, _ = dam_break_solution(200, False)
         solver.bc_lower = [pyclaw.BC.extrap] * 2
         solver.bc_upper = [pyclaw.BC.extrap] * 2
