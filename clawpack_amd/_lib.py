@@ -71,6 +71,8 @@ SIGNATURES = {
     "claw2d_sweep_padded": (C.c_int, [_PD, _VP, _VP, _VP, C.c_double, C.c_int32, _VP, _VP]),
     "claw2d_mass_sum_workspace_bytes": (C.c_size_t, [_PD]),
     "claw2d_mass_sum": (C.c_int, [_PD, _VP, C.c_int32, _VP, _VP, _VP]),
+    "claw2d_mass_sum_noalloc": (C.c_int, [_PD, _VP, C.c_int32, _VP, _VP]),
+    "claw2d_mass_sum_noalloc_max_cells": (C.c_size_t, []),
     "claw2d_adjoint_workspace_bytes": (C.c_size_t, [_PD]),
     "claw2d_adjoint_step": (C.c_int, [_PD, _VP, _VP, _VP, _VP, C.c_double, _VP]),
     "claw2d_batch_prepare": (C.c_int, [_PD, C.POINTER(ClawBatchGrid), C.c_int32, C.c_int32,

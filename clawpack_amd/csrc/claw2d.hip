@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -1462,6 +1463,63 @@ int claw2d_mass_sum(const claw2d_desc* d, const void* q, int32_t m, double* out,
                      m == 0 ? d->depth_ref * (double)n : 0.0);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : hip_fail(e, "mass_sum launch");
+}
+
+// ---- claw2d_mass_sum_noalloc: the mass sum with its partials in a file-scope device array.
+// Each GPU the library runs on holds one copy of `scratch` (kSumBlocks (sum, compensation) pairs,
+// 16 KiB) until the process ends.  Host code never passes the array's name to a kernel (in host
+// code it is a stand-in, not a device address): the device address is looked up once per device
+// with hipGetSymbolAddress and kept in g_scratch_ptr.  Every call on one GPU shares its array, so
+// calls that overlap on one GPU (different streams, or threads) race; mass_partial writes every
+// pair before mass_final reads them, so nothing has to start at zero.
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[2 * kSumBlocks];
+static double* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_for_current_device(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "mass_sum_noalloc: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(-33, "mass_sum_noalloc: device index beyond the library's table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "mass_sum_noalloc: hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+size_t claw2d_mass_sum_noalloc_max_cells(void) { return (size_t)1 << 31; }
+
+int claw2d_mass_sum_noalloc(const claw2d_desc* d, const void* q, int32_t m, double* out, void* stream) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  if (!q || !out) return fail(-22, "null buffer");
+  if (m < 0 || m >= d->num_eqn) return fail(-24, "component out of range");
+  // the partials need kSumBlocks pairs whatever the grid; the cell count is bounded by the
+  // 32-bit row arithmetic of the kernels
+  static_assert(sizeof(scratch) == 2 * kSumBlocks * sizeof(double), "scratch holds every partial pair");
+  const size_t n = (size_t)d->mx * d->my;
+  if (n > claw2d_mass_sum_noalloc_max_cells())
+    return fail(-34, "mass_sum_noalloc: grid larger than 2^31 cells");
+  double* ws = nullptr;
+  rc = scratch_for_current_device(&ws);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (d->dtype == CLAW_FP64)
+    hipLaunchKernelGGL(mass_partial<double>, dim3(kSumBlocks), dim3(kSumThreads), 0, s,
+                       (const double*)q + (size_t)m * n, n, ws);
+  else
+    hipLaunchKernelGGL(mass_partial<float>, dim3(kSumBlocks), dim3(kSumThreads), 0, s,
+                       (const float*)q + (size_t)m * n, n, ws);
+  hipLaunchKernelGGL(mass_final, dim3(1), dim3(64), 0, s, (const double*)ws, out,
+                     m == 0 ? d->depth_ref * (double)n : 0.0);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "mass_sum_noalloc launch");
 }
 
 size_t claw2d_abi_struct_size(int32_t which) {

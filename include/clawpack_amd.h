@@ -17,7 +17,8 @@
  * (NULL = default stream).  Every function returns 0 on success, a negative value for an invalid
  * argument and a positive HIP error code for a runtime failure; claw2d_last_error() describes the
  * last failure of the calling thread.  No function allocates device memory: the caller owns every
- * buffer (state planes, control block, workspaces).
+ * buffer (state planes, control block, workspaces), except the one fixed 16 KiB array that
+ * claw2d_mass_sum_noalloc uses in place of a workspace.
  *
  * State layout on the device: structure of arrays, num_eqn planes of my rows of mx cells, x
  * fastest (plane m, row j, column i at index m*mx*my + j*mx + i), element type double (FP64) or
@@ -208,6 +209,17 @@ size_t claw2d_mass_sum_workspace_bytes(const claw2d_desc* d);
  * No counterpart in the reference (pyclaw sums with numpy in FP64). */
 int claw2d_mass_sum(const claw2d_desc* d, const void* q_dev, int32_t m, double* out_dev,
                     void* workspace_dev, void* stream);
+
+/* claw2d_mass_sum without the workspace argument.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The partial sums live in a 16 KiB array of file-scope device
+ * memory that the library holds on each GPU it runs on until the process ends; this function
+ * allocates nothing.  Calls that overlap on one GPU (other streams, other threads) share that
+ * array and race.  Grids up to claw2d_mass_sum_noalloc_max_cells() cells (2^31); larger ones are
+ * rejected.  Same result, bit for bit, as claw2d_mass_sum. */
+int claw2d_mass_sum_noalloc(const claw2d_desc* d, const void* q_dev, int32_t m, double* out_dev,
+                            void* stream);
+size_t claw2d_mass_sum_noalloc_max_cells(void);
 
 /* ----------------------------------------------------------------------------------------
  * Adjoint (no counterpart in the reference: PyClaw has no adjoint).  Shallow water, FP64.
