@@ -4,8 +4,9 @@ The forward map is q^0 -> q^1 -> ... -> q^N with a *frozen* dt sequence (dt_n fi
 CFL-controlled run), so every step is a fixed function of its input and the chain rule applies
 step by step.  Gradients come from the hand-written reverse step ``claw2d_adjoint_step`` (branches
 of the entropy fix, limiter and transverse split taken as in the forward), not from a tape:
-forward states are recomputed from checkpoints every ``checkpoint_every`` steps (sqrt(N) memory),
-and lambda is propagated backwards in FP64 on the GPU.
+forward states are recomputed from checkpoints every ``checkpoint_every`` steps (sqrt(N) memory;
+the last segment, which the backward reverses first, is kept as the forward computes it), and
+lambda is propagated backwards in FP64 on the GPU.
 
     samples, qN = evolve_frozen(q0, desc, dts, gauges, sample_every=10, checkpoint_every=20)
 
@@ -105,16 +106,24 @@ class FrozenDtEvolve(torch.autograd.Function):
         ckpts = {}
         samples = []
         flat_idx = gauges.to(q0.device)
+        # the states of the last segment are kept as the forward makes them: the backward reverses
+        # that segment first, and would otherwise recompute it from its checkpoint at once
+        last0 = ((n - 1) // checkpoint_every) * checkpoint_every if n else 0
+        tail = []
         for k in range(n):
             if k % checkpoint_every == 0:
                 ckpts[k] = a.clone()
+                if k == last0:
+                    tail.append(ckpts[k])
+            elif k > last0:
+                tail.append(a.clone())
             stepper.step(a, b, dts[k])
             a, b = b, a
             if (k + 1) % sample_every == 0:
                 samples.append(a.view(3, -1)[:, flat_idx])
         out = torch.stack(samples) if samples else a.new_zeros((0, 3, flat_idx.numel()))
         ctx.desc, ctx.dts, ctx.sample_every, ctx.checkpoint_every = desc, dts, sample_every, checkpoint_every
-        ctx.ckpts, ctx.flat_idx, ctx.stepper = ckpts, flat_idx, stepper
+        ctx.ckpts, ctx.flat_idx, ctx.stepper, ctx.tail = ckpts, flat_idx, stepper, tail
         return out, a.clone()
 
     @staticmethod
@@ -146,17 +155,21 @@ class FrozenDtEvolve(torch.autograd.Function):
         seg_starts = sorted(ctx.ckpts.keys())
         for s0 in reversed(seg_starts):
             s1 = min(s0 + K, n)
-            states = [ctx.ckpts[s0]]
-            for k in range(s0, s1 - 1):
-                nxt = torch.empty_like(states[-1])
-                stepper.step(states[-1], nxt, dts[k])
-                states.append(nxt)
+            if ctx.tail and s0 == seg_starts[-1]:
+                states = ctx.tail  # kept by the forward (q^{s0} .. q^{s1-1})
+            else:
+                states = [ctx.ckpts[s0]]
+                for k in range(s0, s1 - 1):
+                    nxt = torch.empty_like(states[-1])
+                    stepper.step(states[-1], nxt, dts[k])
+                    states.append(nxt)
             for k in range(s1 - 1, s0 - 1, -1):
                 rc = rev(dref, C.c_void_p(states[k - s0].data_ptr()), lamp, lamp, wsp, dts[k], sh)
                 if rc:
                     check(rc, "claw2d_adjoint_step")
                 add_direct(lam, k)
             del states
+        ctx.tail = None
         return lam, None, None, None, None, None
 
 
