@@ -113,18 +113,23 @@ __device__ __forceinline__ void sweep_body(const AdjArgs& a, const int blk, cons
     bb = x < r8 ? x * (q8 + 1) + i : r8 * (q8 + 1) + (x - r8) * q8 + i;
   }
   const int wave = __builtin_amdgcn_readfirstlane(bb * kWPBA + (threadIdx.x >> 6));
-  if (wave >= (n_ln + 4) * nseg) return;
+  // The column sweep shares lambda between the waves of a workgroup through LDS (a barrier), so
+  // every one of its waves runs to the end; a wave past the last line or outside the swept lines
+  // computes on clamped, in-bounds data and only stores zeros or nothing.
+  const bool active = wave < (n_ln + 4) * nseg;
+  if (DIR == 0 && !active) return;
   const int li = DIR == 0 ? wave / nseg : wave % (n_ln + 4);
   const int seg = DIR == 0 ? wave - li * nseg : wave / (n_ln + 4);
   const int line = li - 2;                  // -2 .. n_ln + 1
   const int p = seg * kOwnA - 5 + lane;     // position along: lanes 3..60 own
-  const bool own = lane >= 3 && lane < 3 + kOwnA && p <= n_al + 1;
+  const bool own = active && lane >= 3 && lane < 3 + kOwnA && p <= n_al + 1;
   const int ew = a.mx + 4;
   const size_t eplane = (size_t)ew * (size_t)(a.my + 4);
   const size_t eoff = DIR == 0 ? (size_t)(line + 2) * ew + (size_t)(p + 2) : (size_t)(p + 2) * ew + (size_t)(line + 2);
   double* const ext = a.ext + (size_t)(3 * DIR) * eplane;
   // the forward sweeps lines -1 .. n_ln only
-  if (line < -1 || line > n_ln) {
+  const bool swept = line >= -1 && line <= n_ln;
+  if (DIR == 0 && !swept) {
     if (own) {
 #pragma unroll
       for (int m = 0; m < 3; ++m) ext[(size_t)m * eplane + eoff] = 0.0;
@@ -152,7 +157,31 @@ __device__ __forceinline__ void sweep_body(const AdjArgs& a, const int blk, cons
   if (DIR == 0) {
     load_lam<0>(a, p, line, r0); load_lam<0>(a, p, line - 1, rm); load_lam<0>(a, p, line + 1, rp);
   } else {
-    load_lam<1>(a, line, p, r0); load_lam<1>(a, line - 1, p, rm); load_lam<1>(a, line + 1, p, rp);
+    // column line's lambda (strided loads) goes to LDS; the waves of this workgroup hold adjacent
+    // columns at the same positions, so a neighbour column inside the workgroup is read from
+    // there and only the workgroup's two edge columns are loaded again
+    __shared__ double lam_cols[kWPBA][3][64];
+    const int wv = threadIdx.x >> 6;
+    load_lam<1>(a, line, p, r0);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) lam_cols[wv][m][lane] = r0[m];
+    __syncthreads();
+    // the previous (next) wave of the workgroup holds column line - 1 (line + 1) when it is in
+    // the same segment, i.e. when this wave's column is not the first (last) of the segment
+    const bool prev_here = wv > 0 && li > 0;
+    const bool next_here = wv < kWPBA - 1 && li < n_ln + 3 && wave + 1 < (n_ln + 4) * nseg;
+    if (prev_here) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) rm[m] = lam_cols[wv - 1][m][lane];
+    } else {
+      load_lam<1>(a, line - 1, p, rm);
+    }
+    if (next_here) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) rp[m] = lam_cols[wv + 1][m][lane];
+    } else {
+      load_lam<1>(a, line + 1, p, rp);
+    }
   }
   double bLX[3], bLG1[3], bLG2[3], bRX[3], bRG1[3], bRG2[3];
 #pragma unroll
@@ -194,7 +223,7 @@ __device__ __forceinline__ void sweep_body(const AdjArgs& a, const int blk, cons
   if (own) {
     const double c[3] = {o[0], DIR == 0 ? o[1] : o[2], DIR == 0 ? o[2] : o[1]};
 #pragma unroll
-    for (int m = 0; m < 3; ++m) ext[(size_t)m * eplane + eoff] = c[m];
+    for (int m = 0; m < 3; ++m) ext[(size_t)m * eplane + eoff] = swept ? c[m] : 0.0;
   }
 }
 
