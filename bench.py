@@ -64,6 +64,9 @@ def parse():
     p.add_argument("--batch-steps", type=int, default=200)
     p.add_argument("--seq-steps", type=int, default=10)
     p.add_argument("--no-split", action="store_true")
+    p.add_argument("--no-large", action="store_true", help="skip the 8192^2 FP64/FP32 line")
+    p.add_argument("--large-n", type=int, default=8192)
+    p.add_argument("--large-steps", type=int, default=100)
     p.add_argument("--no-euler", action="store_true")
     p.add_argument("--euler-n", type=int, default=2048)
     p.add_argument("--euler-steps", type=int, default=500)
@@ -605,6 +608,16 @@ def main():
                        "tile_shape": r32["tile_shape"], "depth_ref": 1.0}
         if out["autotune"] is not None:
             out["autotune"]["fp32"] = r32["tile_shape_ms"]
+    if not a.no_large:
+        lg = {"workload": "the headline workload at %dx%d (config 2's larger size)" % (a.large_n, a.large_n)}
+        for dt_id, name, bpc in ((0, "fp64", 48.0), (1, "fp32", 24.0)):
+            r = run_gpu(a.large_n, dt_id, a.large_steps, 5, a.seed, prewarm_s=a.prewarm, tune=False)
+            lg[name] = {"cell_updates_per_s": r["cells"] * r["accepted"] / (r["ms"] * 1e-3),
+                        "ms_per_step": r["ms"] / r["accepted"],
+                        "roofline_frac": bpc * r["cells"] * r["attempts"] / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        lg["fp32"]["ratio_vs_fp64"] = lg["fp32"]["cell_updates_per_s"] / lg["fp64"]["cell_updates_per_s"]
+        out["large"] = lg
+        progress("8192 done")
     if not a.no_split:
         sp = {}
         for dt_id, name, bpc in ((0, "fp64", 48.0), (1, "fp32", 24.0)):

@@ -161,3 +161,36 @@ def test_batched_methods_equal_sequential_and_oracle(solver, bcl, bcu, lim, dtyp
         assert c1[g].rejected == st["rejected"]
         err = np.abs(qb - qo).max() / np.abs(q).max()
         assert err <= (1e-12 if dtype == 0 else 1e-4), (g, err)
+
+
+def test_fp32_batch_with_stored_depth_deviation():
+    """FP32 batches store the depth as h - depth_ref as single grids do: batched equals one grid at
+    a time bit for bit, and both stay within FP32 rounding of the FP64 batch (per grid, relative to
+    the grid's own max|q|), closer than the same batch storing h."""
+    from clawpack_amd.batch import BatchedGrids
+    from clawpack_amd.device import make_desc
+    print("case fp32 batch depth_ref", flush=True)
+    grids = problems.batch_workload(12, 21, nmin=40, nmax=140)
+    steps = 30
+    runs = {}
+    for name, dtype, ref, seq in (("fp64", 0, 0.0, False), ("ref", 1, 1.0, False), ("ref_seq", 1, 1.0, True),
+                                  ("h", 1, 0.0, False)):
+        d = make_desc(0, dtype, 8, 8, 1.0, 1.0, 3, 3, bc_lower=(2, 2), bc_upper=(2, 2), grav=1.0,
+                      depth_ref=ref, dt_variable=False)
+        b = BatchedGrids(d, grids)
+        b.init(0.0, 2e-3, target_steps=steps)
+        if seq:
+            b.advance_sequential(steps + 2)
+        else:
+            b.advance(steps + 2)
+        torch.cuda.synchronize()
+        c = b.read_ctrls()
+        assert all(x.numsteps == steps and x.status == 0 for x in c)
+        runs[name] = [b.download(g, c) for g in range(len(grids))]
+    for g in range(len(grids)):
+        assert np.array_equal(runs["ref"][g], runs["ref_seq"][g]), g
+    err = lambda k: max(np.abs(runs[k][g][1:] - runs["fp64"][g][1:]).max() / np.abs(runs["fp64"][g]).max()
+                        for g in range(len(grids)))
+    e_ref, e_h = err("ref"), err("h")
+    print("  momentum error vs fp64: stored h - 1 %.2e, stored h %.2e" % (e_ref, e_h), flush=True)
+    assert e_ref <= 1e-6 and e_ref < e_h
