@@ -50,6 +50,20 @@ def adjoint_step(desc, q, lam_in, dt, lam_out=None, workspace=None, stream=None)
     return lam_out
 
 
+_STREAMS = {}
+
+
+def _stream_pair(device):
+    """Two streams of the default priority, made one after the other and kept for the process:
+    the backward reverses on the first and recomputes checkpoint segments on the second.  The
+    same pair on every call keeps the overlap of the two alike from call to call."""
+    idx = torch.device(device).index
+    idx = torch.cuda.current_device() if idx is None else idx
+    if idx not in _STREAMS:
+        _STREAMS[idx] = (torch.cuda.Stream(device=idx), torch.cuda.Stream(device=idx))
+    return _STREAMS[idx]
+
+
 class _Stepper:
     """Fixed-dt forward steps into caller-owned tensors (no host synchronisation)."""
 
@@ -58,9 +72,9 @@ class _Stepper:
         self.desc = desc
         self.grid = DeviceGrid(desc, device)  # owns a zeroed control block for the CFL atomics
 
-    def step(self, src, dst, dt):
+    def step(self, src, dst, dt, stream=None):
         check(self.lib.claw2d_step(C.byref(self.desc), C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()),
-                                   float(dt), self.grid.ctrl_ptr, _stream_handle(None)), "claw2d_step")
+                                   float(dt), self.grid.ctrl_ptr, _stream_handle(stream)), "claw2d_step")
 
 
 def cfl_dt_sequence(desc, q0_planes, nsteps, dt0=None):
@@ -143,32 +157,73 @@ class FrozenDtEvolve(torch.autograd.Function):
         # per-call checks would only add host time to a host-bound loop
         lib = _lib.load()
         rev = lib.claw2d_adjoint_step
+        # Segments are reversed last to first.  The states of the next segment are recomputed from
+        # its checkpoint on a second stream while this segment's reverse steps run on the first:
+        # at grid sizes where one step fills a fraction of the device the two overlap.  Two
+        # buffers of K - 1 states alternate between the segments; events order each buffer's
+        # refill after the reverse steps that read it, and each segment's reverse steps after its
+        # recompute.  The caller's stream waits for both at the end.
+        main = torch.cuda.current_stream(dev)
+        rs, side = _stream_pair(dev)
+        rs.wait_stream(main)
+        side.wait_stream(main)  # checkpoints and the forward's last steps
         dref, wsp, lamp = C.byref(desc), C.c_void_p(ws.data_ptr()), C.c_void_p(lam.data_ptr())
-        sh = _stream_handle(None)
+        sh = _stream_handle(rs)
 
         def add_direct(lam, step_index):
             # samples taken after step `step_index` (i.e. of q^{step_index})
             if gs is not None and step_index > 0 and step_index % S == 0 and step_index // S - 1 < gs.shape[0]:
                 lam.view(3, -1).index_add_(1, flat_idx, gs[step_index // S - 1])
 
-        add_direct(lam, n)
-        seg_starts = sorted(ctx.ckpts.keys())
-        for s0 in reversed(seg_starts):
-            s1 = min(s0 + K, n)
-            if ctx.tail and s0 == seg_starts[-1]:
-                states = ctx.tail  # kept by the forward (q^{s0} .. q^{s1-1})
-            else:
-                states = [ctx.ckpts[s0]]
-                for k in range(s0, s1 - 1):
-                    nxt = torch.empty_like(states[-1])
-                    stepper.step(states[-1], nxt, dts[k])
-                    states.append(nxt)
-            for k in range(s1 - 1, s0 - 1, -1):
-                rc = rev(dref, C.c_void_p(states[k - s0].data_ptr()), lamp, lamp, wsp, dts[k], sh)
-                if rc:
-                    check(rc, "claw2d_adjoint_step")
-                add_direct(lam, k)
-            del states
+        order = list(reversed(sorted(ctx.ckpts.keys())))
+        bufs = [[torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(K - 1)] for _ in range(2)]
+        ready = [None] * len(order)
+        free = [None, None]
+        have_tail = bool(ctx.tail)
+
+        def span(i):
+            return order[i], min(order[i] + K, n)
+
+        def states_of(i):  # q^{s0} .. q^{s1-1} of segment i (in reverse order)
+            s0, s1 = span(i)
+            if i == 0 and have_tail:
+                return ctx.tail
+            return [ctx.ckpts[s0]] + bufs[i % 2][:s1 - s0 - 1]
+
+        def recompute(i):  # enqueues segment i's recompute on `side`, one step per next()
+            s0, s1 = span(i)
+            st = states_of(i)
+            if free[i % 2] is not None:
+                side.wait_event(free[i % 2])
+            for k in range(s0, s1 - 1):
+                stepper.step(st[k - s0], st[k - s0 + 1], dts[k], stream=side)
+                yield
+            ready[i] = torch.cuda.Event()
+            ready[i].record(side)
+
+        with torch.cuda.stream(rs):
+            add_direct(lam, n)
+            if order and not have_tail:
+                for _ in recompute(0):
+                    pass
+            for i in range(len(order)):
+                s0, s1 = span(i)
+                nxt = recompute(i + 1) if i + 1 < len(order) else iter(())
+                if ready[i] is not None:
+                    rs.wait_event(ready[i])
+                st = states_of(i)
+                for k in range(s1 - 1, s0 - 1, -1):
+                    rc = rev(dref, C.c_void_p(st[k - s0].data_ptr()), lamp, lamp, wsp, dts[k], sh)
+                    if rc:
+                        check(rc, "claw2d_adjoint_step")
+                    add_direct(lam, k)
+                    next(nxt, None)
+                for _ in nxt:
+                    pass
+                free[i % 2] = torch.cuda.Event()
+                free[i % 2].record(rs)
+        main.wait_stream(rs)
+        main.wait_stream(side)
         ctx.tail = None
         return lam, None, None, None, None, None
 

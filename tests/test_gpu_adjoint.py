@@ -191,3 +191,34 @@ def test_inversion_gradient_central_differences_unlimited():
     difference is round-off in the differences, |fd - adjoint| <= 1e-8 |grad|."""
     rel_dir, rel_norm = fd_check(0, ndirs=8)
     assert rel_norm.max() <= 1e-8, rel_norm.max()
+
+
+def test_gradient_independent_of_checkpoint_spacing_and_repeat_backward():
+    """The backward recomputes each segment from its checkpoint on a second stream while the
+    previous segment is reversed; the states it recomputes are the forward's bit for bit, so the
+    gradient may not depend on the spacing (one segment per step, segments that do not divide the
+    step count, one segment for the whole run), and a second backward through the same graph,
+    which has no kept last segment and recomputes it too, gives the same gradient."""
+    mx, my, nsteps = 96, 80, 45
+    eta_true, gauges, dx, dy = problems.inversion_workload(mx, my, seed=5, n_gauges=16)
+    d = desc_for(mx, my, dx, dy, (2, 2), (2, 2))
+    q_true = torch.tensor(np.stack([1.0 + eta_true, 0 * eta_true, 0 * eta_true]), device="cuda")
+    dts = cfl_dt_sequence(d, q_true, nsteps)
+    g = torch.tensor(gauges)
+    with torch.no_grad():
+        obs = evolve_frozen(q_true, d, dts, g, 5, 7)[0][:, 0, :]
+    eta0 = 0.5 * eta_true + 0.01 * np.random.default_rng(4).standard_normal(eta_true.shape)
+    grads = {}
+    for K in (1, 3, 7, 30, 45, 100):
+        print("case checkpoint spacing", K, flush=True)
+        eta = torch.tensor(eta0, device="cuda", requires_grad=True)
+        loss = loss_fn(eta, d, dts, g, obs, 5, K)
+        loss.backward(retain_graph=True)
+        first = eta.grad.clone()
+        eta.grad = None
+        loss.backward()
+        assert torch.equal(eta.grad, first), K
+        grads[K] = first.cpu().numpy()
+    for K, v in grads.items():
+        assert np.array_equal(v, grads[7]), K
+    assert np.abs(grads[7]).max() > 0
