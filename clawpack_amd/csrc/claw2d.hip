@@ -53,9 +53,11 @@ static int hip_fail(hipError_t e, const char* where) {
   g_err = std::string(where) + ": " + hipGetErrorString(e);
   return (int)e;
 }
+#ifndef CLAW_KERNEL_UNIT  // a kernel unit (see below) holds templates instantiated there only
 // error reporting for the library's other translation units (claw2d_last_error reads g_err)
 int set_error(int code, const char* msg) { return fail(code, msg); }
 int set_hip_error(hipError_t e, const char* where) { return hip_fail(e, where); }
+#endif
 
 // ------------------------------------------------------------------------------------------
 // cross-lane helpers: ds_bpermute (LDS crossbar) and DPP wavefront shifts (VALU)
@@ -882,6 +884,7 @@ batch_step(StepArgs<T, P> a, BatchDev b) {
 // One evolve_to_time iteration after the step (upstream pyclaw solver.py): accept/reject,
 // advance t, choose the next dt, clip it to tend.
 
+#ifndef CLAW_KERNEL_UNIT
 // One thread per grid; thread 0 also rearms the work counter of the batched step.
 __global__ void batch_control(const claw2d_batch_grid* grids, claw2d_ctrl* ctrls, int* counter,
                               int ngrids, double cfl_max, double cfl_desired, double dt_max) {
@@ -890,6 +893,7 @@ __global__ void batch_control(const claw2d_batch_grid* grids, claw2d_ctrl* ctrls
   if (g >= ngrids) return;
   control_one(ctrls + g, grids[g].dx, grids[g].dy, cfl_max, cfl_desired, dt_max);
 }
+#endif
 
 // ------------------------------------------------------------------------------------------
 // Compensated FP64 sum of one component (Neumaier), fixed reduction order.
@@ -928,6 +932,7 @@ __global__ void __launch_bounds__(kSumThreads) mass_partial(const T* q, size_t n
   }
 }
 
+#ifndef CLAW_KERNEL_UNIT
 // offset: a constant added once to the sum (the reference depth times the cell count of a state
 // that stores h - depth_ref), as one more compensated term
 __global__ void __launch_bounds__(64) mass_final(const double* part, double* out, double offset) {
@@ -940,7 +945,21 @@ __global__ void __launch_bounds__(64) mass_final(const double* part, double* out
   if (offset != 0.0) two_sum_acc(s, comp, offset);
   out[0] = s + comp;
 }
+#endif
 
+// ------------------------------------------------------------------------------------------
+// Step kernels instantiated in translation units of their own (claw2d_swe32.hip,
+// claw2d_euler64.hip), which are built with another LLVM machine scheduler (clawpack_amd/build.py:
+// FP32 shallow water -2.7% with max-ilp, FP64 Euler -2.6% and no spill with iterative-minreg on
+// MI355X; the same schedulers made other kernels up to 6% slower).  The host code below launches
+// them through their kernel handles, which those units register with their own device code.
+// Scheduling does not change the arithmetic: contraction is decided per source expression.
+extern template __global__ void step2_fused<float, SweP<float>, 4, false, 2, false, 0, false>(
+    StepArgs<float, SweP<float>>);
+extern template __global__ void step2_fused<double, EulerP<double>, 4, false, 2, false, 0, false>(
+    StepArgs<double, EulerP<double>>);
+
+#ifndef CLAW_KERNEL_UNIT
 // ------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------
@@ -1661,3 +1680,6 @@ int claw2d_graph_destroy(void* graph_exec) {
 const char* claw2d_last_error(void) { return g_err.c_str(); }
 
 }  // extern "C"
+#else
+}  // namespace claw
+#endif  // CLAW_KERNEL_UNIT
