@@ -1,6 +1,7 @@
 """Autotuner: measures tile shapes (rows per wavefront, wavefronts per workgroup, launch order),
 caches the winner, and every shape gives the same result bit for bit (the work split and launch
 shape never change the arithmetic)."""
+import collections
 import json
 import os
 
@@ -22,9 +23,15 @@ def test_tune_and_cache(tmp_path, monkeypatch):
     shape, ms = autotune.tune_grid(d, q, 1e-3, candidates=(0, 16, 64, 128), steps=5)
     print("tuned shape", shape, ms)
     name = autotune.shape_name(shape["seg_h"], shape["wg_waves"], shape["launch_order"])
-    assert name in ms and ms[name] == min(ms.values())
-    # every packing and order was measured at the best height
-    assert sum(1 for n in ms if n.startswith("h%d/" % shape["seg_h"])) >= len(autotune.SHAPES)
+    dflt = autotune.shape_name(0, 4, 0)
+    # every packing and order was measured at one height (the best of the first stage), and the
+    # default shape beside them
+    heights = collections.Counter(n.split("/")[0] for n in ms)
+    h2, count = heights.most_common(1)[0]
+    assert count >= len(autotune.SHAPES) and dflt in ms
+    # the default stays unless the fastest of the second stage beats it by more than 1%
+    stage2 = {n: t for n, t in ms.items() if n.startswith(h2 + "/") or n == dflt}
+    assert name == dflt or (ms[name] == min(stage2.values()) and ms[name] < 0.99 * ms[dflt])
     table = json.load(open(os.environ["CLAWPACK_AMD_TUNE_CACHE"]))
     assert list(table.values())[0]["seg_h"] == shape["seg_h"]
     assert autotune.tuned_shape(d) == shape  # served from the cache
