@@ -176,25 +176,31 @@ class FrozenDtEvolve(torch.autograd.Function):
                 lam.view(3, -1).index_add_(1, flat_idx, gs[step_index // S - 1])
 
         order = list(reversed(sorted(ctx.ckpts.keys())))
-        bufs = [[torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(K - 1)] for _ in range(2)]
+        have_tail = bool(ctx.tail)
+        first = 1 if have_tail else 0  # the first segment to recompute (in reverse order)
+        # one buffer per segment to recompute, at most two, taken in turn
+        bufs = [[torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(K - 1)]
+                for _ in range(min(2, len(order) - first))]
         ready = [None] * len(order)
         free = [None, None]
-        have_tail = bool(ctx.tail)
 
         def span(i):
             return order[i], min(order[i] + K, n)
 
+        def buf(i):  # buffer index of segment i >= first
+            return (i - first) % 2
+
         def states_of(i):  # q^{s0} .. q^{s1-1} of segment i (in reverse order)
             s0, s1 = span(i)
-            if i == 0 and have_tail:
+            if i < first:
                 return ctx.tail
-            return [ctx.ckpts[s0]] + bufs[i % 2][:s1 - s0 - 1]
+            return [ctx.ckpts[s0]] + bufs[buf(i)][:s1 - s0 - 1]
 
         def recompute(i):  # enqueues segment i's recompute on `side`, one step per next()
             s0, s1 = span(i)
             st = states_of(i)
-            if free[i % 2] is not None:
-                side.wait_event(free[i % 2])
+            if free[buf(i)] is not None:
+                side.wait_event(free[buf(i)])
             for k in range(s0, s1 - 1):
                 stepper.step(st[k - s0], st[k - s0 + 1], dts[k], stream=side)
                 yield
@@ -203,7 +209,7 @@ class FrozenDtEvolve(torch.autograd.Function):
 
         with torch.cuda.stream(rs):
             add_direct(lam, n)
-            if order and not have_tail:
+            if order and first == 0:
                 for _ in recompute(0):
                     pass
             for i in range(len(order)):
@@ -220,8 +226,9 @@ class FrozenDtEvolve(torch.autograd.Function):
                     next(nxt, None)
                 for _ in nxt:
                     pass
-                free[i % 2] = torch.cuda.Event()
-                free[i % 2].record(rs)
+                if i >= first:
+                    free[buf(i)] = torch.cuda.Event()
+                    free[buf(i)].record(rs)
         main.wait_stream(rs)
         main.wait_stream(side)
         ctx.tail = None
