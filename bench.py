@@ -22,17 +22,42 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
-def profiled_traffic(n):
-    """HBM bytes per launch of the FP64 step kernel from the committed rocprofv3 summary (FETCH_SIZE
-    and WRITE_SIZE scaled by a copy-kernel calibration), if one exists for this size."""
+# MI355X peak engine clock: a wave64 VALU instruction occupies one of a CU's 4 SIMDs for 4 cycles
+VALU_CLOCK_GHZ = 2.4
+
+
+def profiled_value(n, key):
+    """A `key: value` line of the committed rocprofv3 summary of the FP64 step kernel at this size
+    (profiles/step2_fused_swe_fp64_NxN.md), or None."""
     path = os.path.join(ROOT, "profiles", "step2_fused_swe_fp64_%dx%d.md" % (n, n))
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(n):
+    """HBM bytes per launch of the FP64 step kernel from the committed rocprofv3 summary (FETCH_SIZE
+    and WRITE_SIZE scaled by a copy-kernel calibration), if one exists for this size."""
+    return profiled_value(n, "traffic_bytes_per_launch")
+
+
+def valu_ceiling(n, cells, per_attempt_s):
+    """The VALU issue ceiling of the FP64 step kernel beside the HBM roofline: the kernel's wave-level
+    VALU instructions per launch (SQ_INSTS_VALU of the committed profile) issued at one per 4 cycles
+    on every SIMD at the peak clock, against the measured launch time."""
+    import torch
+    wave_instr = profiled_value(n, "valu_wave_instructions_per_launch")
+    if wave_instr is None:
+        return None
+    simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+    ceiling_s = wave_instr * 4.0 / simds / (VALU_CLOCK_GHZ * 1e9)
+    return {"wave_instructions_per_launch": wave_instr, "lane_instructions_per_cell": wave_instr * 64.0 / cells,
+            "issue_ceiling_ms": ceiling_s * 1e3, "frac_of_issue_ceiling": ceiling_s / per_attempt_s,
+            "clock_ghz": VALU_CLOCK_GHZ, "simds": simds}
 
 
 def parse():
@@ -597,6 +622,7 @@ def main():
                      "kernel": "step2_fused<double,SweP<double>,4,false,2> (one launch per attempt)",
                      "bytes_per_cell": bytes_per_cell},
     }
+    out["roofline"]["valu"] = valu_ceiling(a.n, r64["cells"], per_attempt_s)
     copy_gbs = measured_copy_gbs()
     out["roofline"]["measured_copy_gbs"] = copy_gbs
     out["roofline"]["frac_of_measured_copy"] = achieved / copy_gbs
