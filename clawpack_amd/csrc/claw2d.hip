@@ -200,8 +200,12 @@ __device__ __forceinline__ typename P::Cell prep_cell(const T q[P::NEQ], T b, co
 // dt/dx along the interface normal and rt the transverse dt/dy that step2.f90 applies to gadd.
 // For an f-wave policy amdq = sum sf_p alpha_p e_p, apdq = sum (1 - sf_p) alpha_p e_p and the
 // correction carries sign(s_p) where the wave form has |s_p| (flux2.f90 with use_fwave).
-// alL/mL and alR/mR are the wave strengths of the interfaces below/above along the sweep and
-// the eigenvector products e_p(neighbour).e_p(this) the limiter needs.
+// upL[p] is the limiter's upwind term from the interface below/left along the sweep, alpha_p of
+// that interface times e_p(neighbour).e_p(this), formed where the product is known; alR/mR are the
+// wave strengths of the interface above/right and their eigenvector products.  (Given as one
+// product, the left term reaches the select already multiplied: passed as alpha and m, the
+// compiler rewrote select(s > 0, aL*mL, aR*mR) as select(aL, aR)*select(mL, mR), two selects more
+// per wave.)
 // TWC >= 0: the method is known at compile time (order 2, transverse_waves = TWC), so the
 // order/transverse selects compile away; TWC < 0: order and transverse_waves from a.
 // CAPA (capacity form, step2.f90/flux2.f90 with mcapa > 0): rkL/rkR are 1/kappa of the left and
@@ -211,9 +215,8 @@ __device__ __forceinline__ typename P::Cell prep_cell(const T q[P::NEQ], T b, co
 // (dt/dx)(1/kappa_L + 1/kappa_R)/2 in place of dt/dx, and the transverse terms of amdq (apdq)
 // carry dt/(dx kappa) of the left (right) cell.
 template <class T, class P, int LIM, int TWC, bool CAPA = false>
-__device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T alL[P::NW],
-                                             const T mL[P::NW], const T alR[P::NW],
-                                             const T mR[P::NW], T r, T rt,
+__device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T upL[P::NW],
+                                             const T alR[P::NW], const T mR[P::NW], T r, T rt,
                                              const StepArgs<T, P>& a, T LX[P::NEQ], T LG1[P::NEQ],
                                              T LG2[P::NEQ], T RX[P::NEQ], T RG1[P::NEQ],
                                              T RG2[P::NEQ], T rkL = T(1), T rkR = T(1)) {
@@ -231,8 +234,8 @@ __device__ __forceinline__ void finish_iface(const typename P::Iface& f, const T
     const T e = sf * al, q = P::FWAVE ? al - e : (s - sf) * al;
     T k = T(0);
     if (second) {
-      const T aL = alL[p], aR = alR[p], ml = mL[p], mr = mR[p];  // rvalues: no lvalue select
-      const T t = (s > T(0) ? aL * ml : aR * mr) * in[p];
+      const T aL = upL[p], aR = alR[p], mr = mR[p];  // rvalues: no lvalue select
+      const T t = (s > T(0) ? aL : aR * mr) * in[p];
       const int meth = LIM > 0 ? LIM : a.mthlim[p];
       const T sa = vabs(s);  // |s| as a source modifier (a select form costs two instructions)
       const T sw = P::FWAVE ? copysign(T(1), s) : sa;
@@ -410,7 +413,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
   bool bad = false;
   Cell cprev = {};
   Iface fyprev;
-  T yal2[NW], ymprev[NW];  // y: strengths of interface R-2, e-products of the pair (R-2, R-1)
+  T yup[NW];  // y: alpha_p of interface R-2 times e_p(R-2).e_p(R-1), the upwind term of R-1
   T qm2[NEQ], qnext[NEQ];  // q of row R-2 (the row being finished), prefetched row R+1
   T acc0[NEQ], acc1[NEQ], acc2[NEQ], acc3[NEQ];
   T yX[NEQ], yG1[NEQ], yG2[NEQ];
@@ -421,7 +424,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
     yX[m] = yG1[m] = yG2[m] = T(0);
   }
 #pragma unroll
-  for (int p = 0; p < NW; ++p) yal2[p] = ymprev[p] = T(0);
+  for (int p = 0; p < NW; ++p) yup[p] = T(0);
 
   // the x-sweep alone needs only the owned rows; every other form marches two halo rows each side
   const int rlo = SWEEP == 1 ? y0 : y0 - 2, rhi = SWEEP == 1 ? y1 - 1 : y1 + 1;
@@ -488,7 +491,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         P::mdot(lp, ln, mR);
         if (ST || R >= y0 + 1) {
           T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-          finish_iface<T, P, LIM, TWC, CAPA>(fyprev, yal2, ymprev, fynew.al, mR, dtdy, dtdx, a, LX, LG1,
+          finish_iface<T, P, LIM, TWC, CAPA>(fyprev, yup, fynew.al, mR, dtdy, dtdx, a, LX, LG1,
                                              LG2, RX, RG1, RG2, rkm2, rkprev);
           yswap<NEQ>(LX); yswap<NEQ>(LG1); yswap<NEQ>(LG2);
           yswap<NEQ>(RX); yswap<NEQ>(RG1); yswap<NEQ>(RG2);
@@ -533,8 +536,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
         }
   #pragma unroll
         for (int p = 0; p < NW; ++p) {
-          ymprev[p] = mR[p];
-          yal2[p] = fyprev.al[p];
+          yup[p] = P::unit_m(p) ? fyprev.al[p] : fyprev.al[p] * mR[p];
         }
       }
 
@@ -562,7 +564,7 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
       // forms the eigenvector products m_p of its pair (this, right) and hands the right
       // neighbour the product it needs when its wave p is right-going, alpha_p(this) * m_p, which
       // is its left partner's upwind term (NW values instead of the left partner's NL).
-      T lself[NL], lnb[NL], mR[NW], mL[NW], alR[NW], alL[NW];
+      T lself[NL], lnb[NL], mR[NW], alR[NW], upL[NW];
       P::lim(fxi, lself);
 #pragma unroll
       for (int k = 0; k < NL; ++k) lnb[k] = from_right<DPP, T, DPPZ>(lself[k]);
@@ -570,12 +572,11 @@ __device__ __forceinline__ void march_item(const StepArgs<T, P>& a, const T* __r
 #pragma unroll
       for (int p = 0; p < NW; ++p) {
         const T up = P::unit_m(p) ? lself[p] : lself[p] * mR[p];
-        alL[p] = from_left<DPP, T, DPPZ>(up);
-        mL[p] = T(1);
+        upL[p] = from_left<DPP, T, DPPZ>(up);
         alR[p] = lnb[p];
       }
       T LX[NEQ], LG1[NEQ], LG2[NEQ], RX[NEQ], RG1[NEQ], RG2[NEQ];
-      finish_iface<T, P, LIM, TWC, CAPA>(fxi, alL, mL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2,
+      finish_iface<T, P, LIM, TWC, CAPA>(fxi, upL, alR, mR, dtdx, dtdy, a, LX, LG1, LG2, RX, RG1, RG2,
                                          rkLx, rkX);
       if constexpr (SPLIT) {
         // q* of row R: the x-sweep only (no transverse terms); valid on the owned columns
