@@ -114,7 +114,7 @@ typedef struct claw2d_ctrl {
   int32_t done;
   int32_t status;                  /* CLAW_STATUS_* bits */
   int32_t flags;                   /* CLAW_FLAG_* bits */
-  int32_t arrive;                  /* wavefronts of the running step that have finished; 0 between calls */
+  int32_t arrive;                  /* workgroups of the running step that have finished; 0 between calls */
   int32_t reserved;
 } claw2d_ctrl;
 
