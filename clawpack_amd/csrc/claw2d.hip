@@ -38,10 +38,10 @@ namespace claw {
 constexpr int kLanes = 64;
 constexpr int kOwn = 60;  // owned columns per strip
 constexpr int kWPB = 4;   // independent wavefronts per workgroup
-// FP64 shallow-water steady rows per segment from which issue priority follows progress
-// (shorter segments ran 0.5-2% slower with it on MI355X: 2048^2 and below)
+// Shallow-water steady rows per segment from which issue priority follows progress (shorter
+// segments ran 0.5-2% slower with it on MI355X: 2048^2 and below)
 constexpr int kPrioRows = 64;
-// the same for the FP64 Euler kernel (about twice the work per row): marched rows per segment
+// the same for the Euler kernels (about twice the work per row): marched rows per segment
 constexpr int kPrioRowsEuler = 16;
 
 static thread_local std::string g_err;
@@ -652,25 +652,25 @@ __device__ __forceinline__ ItemMax march_item(const StepArgs<T, P>& a, const T* 
   if constexpr (SWEEP == 0 && NEQ == 3) {
     int R = rlo;
     for (; R < y0 + 2 && R <= rhi; ++R) row(R, std::false_type{});
-    // Issue priority from progress (FP64, segments of at least kPrioRows steady rows): 3 over the
-    // first quarter of the steady rows, then 2, 1 and 0.  The wavefronts that share a SIMD start
+    // Issue priority from progress (segments of at least kPrioRows steady rows): 3 over the first
+    // quarter of the steady rows, then 2, 1 and 0.  The wavefronts that share a SIMD start
     // together with equal work, and the arbiter otherwise favours the older one: it finishes
-    // early and the other runs its last stretch alone, at one wavefront per SIMD.  With the
-    // priority falling as a wavefront advances, the one behind is issued first, and the pair
-    // finishes closer together.  Issue order only: no result bit changes.
-    const bool prio = sizeof(T) == 8 && y1 - R >= kPrioRows;
+    // early and the others run their last stretch with fewer wavefronts per SIMD.  With the
+    // priority falling as a wavefront advances, the one behind is issued first, and they finish
+    // closer together.  Issue order only: no result bit changes.  (For FP32 it measured slower
+    // while every wavefront still sent its own device-scope atomics at its end: finishing together
+    // queued them; with one set per workgroup it is 7% faster at 4096^2, 10% at 8192^2.)
+    const bool prio = y1 - R >= kPrioRows;
     const int pq = max((y1 - R) >> 2, 1);
     int Rq = prio ? R + pq : INT_MAX, pl = 3;  // next row at which the priority steps down
     if (prio) __builtin_amdgcn_s_setprio(3);
     auto prio_at = [&](int r) {
-      if constexpr (sizeof(T) == 8) {
-        if (r >= Rq) {
-          Rq += pq;
-          --pl;
-          if (pl == 2) __builtin_amdgcn_s_setprio(2);
-          else if (pl == 1) __builtin_amdgcn_s_setprio(1);
-          else __builtin_amdgcn_s_setprio(0);
-        }
+      if (r >= Rq) {
+        Rq += pq;
+        --pl;
+        if (pl == 2) __builtin_amdgcn_s_setprio(2);
+        else if (pl == 1) __builtin_amdgcn_s_setprio(1);
+        else __builtin_amdgcn_s_setprio(0);
       }
     };
     if constexpr (SPLIT && sizeof(T) == 8) {
@@ -689,15 +689,14 @@ __device__ __forceinline__ ItemMax march_item(const StepArgs<T, P>& a, const T* 
     if (prio) __builtin_amdgcn_s_setprio(0);
     for (; R <= rhi; ++R) row(R, std::false_type{});
   } else {
-    // Issue priority from progress, as above, over every marched row of an FP64 Euler segment
-    // (2048^2: -5% on MI355X; the FP32 Euler kernel, at 3 wavefronts per SIMD, ran 4% slower
-    // with it)
-    const bool prio = sizeof(T) == 8 && NEQ == 4 && rhi - rlo >= kPrioRowsEuler;
+    // Issue priority from progress, as above, over every marched row of an Euler segment (2048^2
+    // on MI355X: FP64 -5%, FP32 -10%)
+    const bool prio = NEQ == 4 && rhi - rlo >= kPrioRowsEuler;
     const int pq = max((rhi + 1 - rlo) >> 2, 1);
     int Rq = prio ? rlo + pq : INT_MAX, pl = 3;
     if (prio) __builtin_amdgcn_s_setprio(3);
     for (int R = rlo; R <= rhi; ++R) {
-      if constexpr (sizeof(T) == 8 && NEQ == 4) {
+      if constexpr (NEQ == 4) {
         if (R >= Rq) {
           Rq += pq;
           --pl;
