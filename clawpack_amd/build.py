@@ -35,6 +35,8 @@ SOURCES = [
     ("claw2d_swe32.hip", ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]),
     # FP64 Euler (MC, transverse_waves 2): -2.6%, no spill
     ("claw2d_euler64.hip", ["-mllvm", "-amdgpu-sched-strategy=iterative-minreg"]),
+    # FP32 Euler (MC, transverse_waves 2): -1.1%
+    ("claw2d_euler32.hip", ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]),
 ]
 
 

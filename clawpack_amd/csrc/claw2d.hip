@@ -998,15 +998,18 @@ __global__ void __launch_bounds__(64) mass_final(const double* part, double* out
 
 // ------------------------------------------------------------------------------------------
 // Step kernels instantiated in translation units of their own (claw2d_swe32.hip,
-// claw2d_euler64.hip), which are built with another LLVM machine scheduler (clawpack_amd/build.py:
-// FP32 shallow water -2.7% with max-ilp, FP64 Euler -2.6% and no spill with iterative-minreg on
-// MI355X; the same schedulers made other kernels up to 6% slower).  The host code below launches
-// them through their kernel handles, which those units register with their own device code.
-// Scheduling does not change the arithmetic: contraction is decided per source expression.
+// claw2d_euler64.hip, claw2d_euler32.hip), which are built with another LLVM machine scheduler
+// (clawpack_amd/build.py: FP32 shallow water -2.7% and FP32 Euler -1.1% with max-ilp, FP64 Euler
+// -2.6% and no spill with iterative-minreg on MI355X; the same schedulers made other kernels up to
+// 6% slower).  The host code below launches them through their kernel handles, which those units
+// register with their own device code.  Scheduling does not change the arithmetic: contraction is
+// decided per source expression.
 extern template __global__ void step2_fused<float, SweP<float>, 4, false, 2, false, 0, false>(
     StepArgs<float, SweP<float>>);
 extern template __global__ void step2_fused<double, EulerP<double>, 4, false, 2, false, 0, false>(
     StepArgs<double, EulerP<double>>);
+extern template __global__ void step2_fused<float, EulerP<float>, 4, false, 2, false, 0, false>(
+    StepArgs<float, EulerP<float>>);
 
 #ifndef CLAW_KERNEL_UNIT
 // ------------------------------------------------------------------------------------------
