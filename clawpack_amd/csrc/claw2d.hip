@@ -110,11 +110,12 @@ template <bool DPP, class T, bool ZERO = false> __device__ __forceinline__ T fro
     return shfl(v, (lane < 63 ? lane + 1 : 63) * 4);
   }
 }
-// Which form a step kernel uses (measured on MI355X): DPP for FP64 and for the Euler policy, where
-// at 2-3 wavefronts per SIMD the ds_bpermute latency is on the critical path (FP64 shallow water
-// -4%, Euler FP64 -9%, Euler FP32 -7%); ds_bpermute for FP32 shallow water, which at 4 wavefronts
-// per SIMD hides that latency and is bound by VALU issue, where the DPP moves cost 8%.
-template <class T, class P> constexpr bool kDppShift = sizeof(T) == 8 || P::NEQ == 4;
+// Which form a step kernel uses (measured on MI355X): DPP for FP64, where at 2 wavefronts per SIMD
+// the ds_bpermute latency is on the critical path (shallow water -4.7%, Euler -9%); ds_bpermute
+// for FP32.  A DPP move issues like an FP64 instruction, one wave64 per 4 cycles, while FP32
+// arithmetic issues in 2, so a 32-bit shift costs an FP32 kernel two arithmetic slots, and at 3-4
+// wavefronts per SIMD the crossbar's latency is hidden (FP32 shallow water -8%, FP32 Euler -5.9%).
+template <class T, class P> constexpr bool kDppShift = sizeof(T) == 8;
 
 // max that lets a NaN through (so a NaN speed poisons the CFL number instead of vanishing)
 template <class T> __device__ __forceinline__ T nmax(T acc, T s) { return (s > acc || s != s) ? s : acc; }
