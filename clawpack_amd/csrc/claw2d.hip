@@ -477,13 +477,17 @@ __device__ __forceinline__ ItemMax march_item(const StepArgs<T, P>& a, const T* 
     const Cell cR = prep_cell<P>(qR, bR, a.par);
 
     // ---- request the left neighbour's cell data for the x-interfaces of row R now, so that the
-    // y work below hides the ds_bpermute latency
+    // y work below hides the ds_bpermute latency.  Shallow water sends it over ds_bpermute in
+    // FP64 too: the y work hides the crossbar's latency there, and the 14 DPP moves it replaces
+    // issue at the FP64 rate (4096^2 -2.5%; the Euler kernel's shorter y work does not hide it,
+    // +0.6%, so Euler keeps DPP here).
     T pl[NC];
     if constexpr (SWEEP != 2) {
+      constexpr bool kCellDpp = DPP && P::NEQ == 4;
       T pk[NC];
       P::pack(cR, pk);
 #pragma unroll
-      for (int k = 0; k < NC; ++k) pl[k] = from_left<DPP, T, DPPZ>(pk[k]);
+      for (int k = 0; k < NC; ++k) pl[k] = from_left<kCellDpp, T, DPPZ>(pk[k]);
     }
     T rkLx = T(1);  // 1/kappa of the left cell of the row's x-interface
     if constexpr (CAPA && SWEEP != 2) rkLx = from_left<DPP, T, DPPZ>(rkX);
