@@ -479,11 +479,11 @@ __device__ __forceinline__ ItemMax march_item(const StepArgs<T, P>& a, const T* 
     // ---- request the left neighbour's cell data for the x-interfaces of row R now, so that the
     // y work below hides the ds_bpermute latency.  Shallow water sends it over ds_bpermute in
     // FP64 too: the y work hides the crossbar's latency there, and the 14 DPP moves it replaces
-    // issue at the FP64 rate (4096^2 -2.5%; the Euler kernel's shorter y work does not hide it,
-    // +0.6%, so Euler keeps DPP here).
+    // issue at the FP64 rate (4096^2 -2.5%).  The Euler and bathymetry kernels keep DPP here:
+    // their y work does not hide it (Euler +0.6%, bathymetry +0.5%, split +1%).
     T pl[NC];
     if constexpr (SWEEP != 2) {
-      constexpr bool kCellDpp = DPP && P::NEQ == 4;
+      constexpr bool kCellDpp = DPP && (P::NEQ == 4 || P::FWAVE);
       T pk[NC];
       P::pack(cR, pk);
 #pragma unroll
