@@ -6,8 +6,8 @@
 //   adj_sweeps    both sweep directions in one launch (they are independent).  x part: one
 //                 wavefront per (row line, segment of 58 columns); lane = column = the x-interface
 //                 on its left.  Each lane computes its interface's Roe split once; the limiter
-//                 partners' data and, in reverse, their adjoints travel one lane left or right over
-//                 ds_bpermute; the cell adjoint of column c is the sum of the shares of interfaces
+//                 partners' data and, in reverse, their adjoints travel one lane left or right by
+//                 DPP wavefront rotations; the cell adjoint of column c is the sum of the shares of interfaces
 //                 c and c+1.  It writes the x-sweep share of every extended cell (interior + 2
 //                 ghost layers) to the workspace's first three (my+4) x (mx+4) planes.  y part: the
 //                 same along columns (lane = row position, strided loads; the waves of a
@@ -34,16 +34,31 @@ namespace adj {
 constexpr int kOwnA = 58;  // owned positions per wavefront (3 halo lanes left, 3 right)
 constexpr int kWPBA = 4;   // wavefronts per workgroup
 
-__device__ __forceinline__ double bperm(double v, int addr) {
+// One-lane exchanges as DPP wavefront rotations (VALU moves, no LDS round trip): rotl gives lane i
+// the value of lane i-1 and lane 0 that of lane 63; rotr gives lane i the value of lane i+1 and
+// lane 63 that of lane 0.  At the sweep kernel's 2 wavefronts per SIMD the crossbar's latency sat
+// on the dependency chain: a reverse step ran 3% faster with these than with ds_bpermute.  Every
+// call site is in wave-uniform control flow (all 64 lanes active).
+template <int CTRL> __device__ __forceinline__ double dpp_rot(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffLL));
-  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)rhi << 32) | (unsigned int)rlo);
 }
-__device__ __forceinline__ LimD bperm(const LimD& d, int addr) {
+constexpr int kDppWaveRor1 = 0x13C, kDppWaveRol1 = 0x134;
+__device__ __forceinline__ double rotl(double v) { return dpp_rot<kDppWaveRor1>(v); }
+__device__ __forceinline__ double rotr(double v) { return dpp_rot<kDppWaveRol1>(v); }
+__device__ __forceinline__ LimD rotl(const LimD& d) {
   LimD o;
-  o.a1 = bperm(d.a1, addr); o.a2 = bperm(d.a2, addr); o.a3 = bperm(d.a3, addr);
-  o.s1 = bperm(d.s1, addr); o.s3 = bperm(d.s3, addr); o.v = bperm(d.v, addr);
+  o.a1 = rotl(d.a1); o.a2 = rotl(d.a2); o.a3 = rotl(d.a3);
+  o.s1 = rotl(d.s1); o.s3 = rotl(d.s3); o.v = rotl(d.v);
+  return o;
+}
+__device__ __forceinline__ LimD rotr(const LimD& d) {
+  LimD o;
+  o.a1 = rotr(d.a1); o.a2 = rotr(d.a2); o.a3 = rotr(d.a3);
+  o.s1 = rotr(d.s1); o.s3 = rotr(d.s3); o.v = rotr(d.v);
   return o;
 }
 
@@ -136,7 +151,6 @@ __device__ __forceinline__ void sweep_body(const AdjArgs& a, const int blk, cons
     }
     return;
   }
-  const int aL = ((lane + 63) & 63) * 4, aR = ((lane + 1) & 63) * 4;
   const double r = DIR == 0 ? a.dtdx : a.dtdy, rt = DIR == 0 ? a.dtdy : a.dtdx;
 
   // ---- forward: cells p-1 | p, the Roe split and entropy fix of interface p
@@ -144,13 +158,13 @@ __device__ __forceinline__ void sweep_body(const AdjArgs& a, const int blk, cons
   if (DIR == 0) load_q<0>(a, p, line, qo); else load_q<1>(a, line, p, qo);
   double ql[3];
 #pragma unroll
-  for (int m = 0; m < 3; ++m) ql[m] = bperm(qo[m], aL);
+  for (int m = 0; m < 3; ++m) ql[m] = rotl(qo[m]);
   const CellF cR = cell_fwd(qo[0], qo[1], qo[2]);
   const CellF cL = cell_fwd(ql[0], ql[1], ql[2]);
   const WaveF w = wave_fwd(cL, cR, a.M.g);
   const FixF fx = fix_fwd(cL, cR, w, a.M.g, a.M.sqg);
   const LimD me = lim_of(w);
-  const LimD nl = bperm(me, aL), nr = bperm(me, aR);
+  const LimD nl = rotl(me), nr = rotr(me);
 
   // ---- adjoints of the six output cells: (p-1 | p) x (line-1, line, line+1)
   double r0[3], rm[3], rp[3];
@@ -186,7 +200,7 @@ __device__ __forceinline__ void sweep_body(const AdjArgs& a, const int blk, cons
   double bLX[3], bLG1[3], bLG2[3], bRX[3], bRG1[3], bRG2[3];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    const double l0 = bperm(r0[m], aL), lm = bperm(rm[m], aL), lp = bperm(rp[m], aL);
+    const double l0 = rotl(r0[m]), lm = rotl(rm[m]), lp = rotl(rp[m]);
     bLX[m] = l0;
     bLG1[m] = l0 - lm;
     bLG2[m] = lp - l0;
@@ -206,18 +220,18 @@ __device__ __forceinline__ void sweep_body(const AdjArgs& a, const int blk, cons
   zero(Rb);
   iface_core(cL, cR, w, fx, nl, nr, bLX, bLG1, bLG2, bRX, bRG1, bRG2, r, rt, a.M, Wb, NLb, NRb, Lb, Rb);
   // this interface is the right partner of interface p-1 and the left partner of p+1
-  add_lim(Wb, bperm(NRb, aL));
-  add_lim(Wb, bperm(NLb, aR));
+  add_lim(Wb, rotl(NRb));
+  add_lim(Wb, rotr(NLb));
   wave_rev(cL, cR, w, a.M.g, Wb, Lb, Rb);
   // cell p: right cell of interface p, left cell of interface p+1
   CellB tot = Rb;
-  tot.h += bperm(Lb.h, aR);
-  tot.hn += bperm(Lb.hn, aR);
-  tot.ht += bperm(Lb.ht, aR);
-  tot.s += bperm(Lb.s, aR);
-  tot.r += bperm(Lb.r, aR);
-  tot.P += bperm(Lb.P, aR);
-  tot.Q += bperm(Lb.Q, aR);
+  tot.h += rotr(Lb.h);
+  tot.hn += rotr(Lb.hn);
+  tot.ht += rotr(Lb.ht);
+  tot.s += rotr(Lb.s);
+  tot.r += rotr(Lb.r);
+  tot.P += rotr(Lb.P);
+  tot.Q += rotr(Lb.Q);
   double o[3];
   cell_rev(cR, tot, o);
   if (own) {
