@@ -537,13 +537,16 @@ __device__ __forceinline__ ItemMax march_item(const StepArgs<T, P>& a, const T* 
           if (ST || R >= y0 + 2) {
             // cell (c, R-2) has all its y-contributions
             if constexpr (!kImmY) {
+              // The split shallow-water step (3 wavefronts per SIMD) hides the crossbar's latency
+              // here and saves the DPP moves (-1%); the unsplit kernels at 2 per SIMD do not.
+              constexpr bool kYDpp = DPP && !(SPLIT && P::NEQ == 3 && !P::FWAVE);
   #pragma unroll
               for (int m = 0; m < NEQ; ++m) {
                 const T Y = yX[m] + LX[m];
                 const T G1 = yG1[m] + LG1[m];
                 const T G2 = yG2[m] + LG2[m];
-                const T g2l = from_left<DPP, T, DPPZ>(G2);
-                const T g1r = from_right<DPP, T, DPPZ>(G1);
+                const T g2l = from_left<kYDpp, T, DPPZ>(G2);
+                const T g1r = from_right<kYDpp, T, DPPZ>(G1);
                 acc0[m] += ((Y + G1) - G2) + (g2l - g1r);
               }
             }
