@@ -528,8 +528,11 @@ __device__ __forceinline__ ItemMax march_item(const StepArgs<T, P>& a, const T* 
             // both cells' shares go into their accumulators now (rows R-2 and R-1)
   #pragma unroll
             for (int m = 0; m < NEQ; ++m) {
-              const T l2 = from_left<DPP, T, DPPZ>(LG2[m]), l1 = from_right<DPP, T, DPPZ>(LG1[m]);
-              const T r2 = from_left<DPP, T, DPPZ>(RG2[m]), r1 = from_right<DPP, T, DPPZ>(RG1[m]);
+              // (TWC 0: zero transverse shares, no exchange, as below)
+              const T l2 = TWC == 0 ? T(0) : from_left<DPP, T, DPPZ>(LG2[m]);
+              const T l1 = TWC == 0 ? T(0) : from_right<DPP, T, DPPZ>(LG1[m]);
+              const T r2 = TWC == 0 ? T(0) : from_left<DPP, T, DPPZ>(RG2[m]);
+              const T r1 = TWC == 0 ? T(0) : from_right<DPP, T, DPPZ>(RG1[m]);
               acc0[m] += ((LX[m] + LG1[m]) - LG2[m]) + (l2 - l1);
               acc1[m] += ((RX[m] + RG1[m]) - RG2[m]) + (r2 - r1);
             }
@@ -537,16 +540,16 @@ __device__ __forceinline__ ItemMax march_item(const StepArgs<T, P>& a, const T* 
           if (ST || R >= y0 + 2) {
             // cell (c, R-2) has all its y-contributions
             if constexpr (!kImmY) {
-              // The split shallow-water step (3 wavefronts per SIMD) hides the crossbar's latency
-              // here and saves the DPP moves (-1%); the unsplit kernels at 2 per SIMD do not.
-              constexpr bool kYDpp = DPP && !(SPLIT && P::NEQ == 3 && !P::FWAVE);
   #pragma unroll
               for (int m = 0; m < NEQ; ++m) {
                 const T Y = yX[m] + LX[m];
                 const T G1 = yG1[m] + LG1[m];
                 const T G2 = yG2[m] + LG2[m];
-                const T g2l = from_left<kYDpp, T, DPPZ>(G2);
-                const T g1r = from_right<kYDpp, T, DPPZ>(G1);
+                // without transverse propagation (TWC 0: the split step) the transverse shares
+                // are zero and nothing crosses lanes; the compiler does not fold a DPP move of a
+                // constant zero, so the exchange is left out here
+                const T g2l = TWC == 0 ? T(0) : from_left<DPP, T, DPPZ>(G2);
+                const T g1r = TWC == 0 ? T(0) : from_right<DPP, T, DPPZ>(G1);
                 acc0[m] += ((Y + G1) - G2) + (g2l - g1r);
               }
             }
